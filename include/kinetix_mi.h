@@ -1,0 +1,121 @@
+/*
+ * kinetix_mi.h -- C ABI of the MI355X-native (gfx950) KinetiX hot path.
+ *
+ * Drop-in for the reference's host API benchmark/src/kinetix.hpp:17-107 (namespace
+ * kinetix, C++ with occa::device/occa::memory arguments). Here every argument is a plain
+ * pointer or size: device buffers are HIP device pointers, the stream is a hipStream_t
+ * passed as void* (NULL = the default stream). Every entry point returns an int status
+ * (KX_OK = 0, negative on error) and never throws across the ABI; launches are
+ * asynchronous on the given stream, as the reference's OCCA launches are
+ * (benchmark/src/bk.cpp:697-716 synchronises with device.finish()).
+ *
+ * Device-side layout (benchmark/src/bk.cpp:606-648, benchmark/okl/productionRates.okl:1-8):
+ * structure of arrays, FP64,
+ *     state[id]                       = T / T_ref
+ *     state[id + offsetT + k*offset]  = Y_k               (k = 0 .. n_species-1)
+ * with offsetT >= n_states and offset >= n_states (offsetT = offset = n_states for a
+ * dense array; larger values address a sub-view of a bigger SoA buffer, the state-sharding
+ * idiom of the reference's offset arguments).
+ */
+#ifndef KINETIX_MI_H
+#define KINETIX_MI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  KX_OK = 0,
+  KX_ERR_INVALID_ARG = -1,
+  KX_ERR_NOT_INITIALIZED = -2,   /* handle unknown or finalised */
+  KX_ERR_NOT_BUILT = -3,         /* kx_build not called (reference: assert(initialized)) */
+  KX_ERR_HIP = -4,               /* a HIP runtime call failed; kx_last_error() has the text */
+  KX_ERR_IO = -5,                /* mechanism artefacts missing or unreadable */
+  KX_ERR_NO_KERNEL = -6,         /* the code object lacks the requested kernel variant */
+  KX_ERR_TOO_LARGE = -7          /* request exceeds a documented size limit */
+};
+
+typedef int kx_mech_t;           /* handle of one loaded mechanism on one device */
+
+/* Human-readable text of the last error on this thread. */
+const char *kx_last_error(void);
+
+/*
+ * kinetix::init (kinetix.cpp:481-599): load one mechanism's generated artefacts
+ * (mech_dir/kernels.hsaco + mech_dir/mech.txt, written by `python -m kinetix_amd.gen`)
+ * onto HIP device `device_id`. The reference runs its generator and JIT-compiles OKL here;
+ * this build generates and compiles ahead of time, so init only loads the code object.
+ */
+int kx_init(const char *mech_dir, int device_id, kx_mech_t *out);
+
+/* kinetix::build (kinetix.cpp:609-784): reference pressure [Pa], temperature [K] and mass
+ * fractions (n_species values); the state arrays hold T/T_ref and p arguments are p/p_ref. */
+int kx_build(kx_mech_t m, double ref_pressure, double ref_temperature,
+             const double *ref_mass_fractions);
+
+int kx_finalize(kx_mech_t m);
+
+/* kinetix::productionRates (kinetix.cpp:786-813) -> benchmark/okl/productionRates.okl.
+ * pressure = p/p_ref (scalar for all states). rates: (n_species+1) rows in the state
+ * layout; rates[id] = heat release rate [W/m^3], rates[id+offsetT+k*offset] = W_k wdot_k
+ * [kg/m^3/s]. FP64 throughout. */
+int kx_production_rates(kx_mech_t m, int n_states, int offsetT, int offset, double pressure,
+                        const double *d_state, double *d_rates, void *stream);
+
+/* Mixed-precision variant of kx_production_rates (no reference counterpart; the nearest is
+ * the reference's "fpmix" kernels, kinetix.cpp:254-281, which run ALL kinetics in FP32):
+ * Arrhenius/falloff/third-body in FP32, equilibrium constants, net forward-minus-reverse
+ * rates and the per-species sums in FP64, stoichiometric accumulation q.nu on FP32 MFMA.
+ * Same layout and units as kx_production_rates. */
+int kx_production_rates_mixed(kx_mech_t m, int n_states, int offsetT, int offset,
+                              double pressure, const double *d_state, double *d_rates,
+                              void *stream);
+
+/* kinetix::mixtureAvgTransportProps (kinetix.cpp:815-841) -> transportProps.okl.
+ * Argument order follows the reference API: viscosity [Pa s], conductivity [W/m/K],
+ * rhoD (n_species rows, rho*D_km [kg/m/s], at [k*offset + id]). pressure is passed
+ * through as the reference does (it cancels in rho*D). */
+int kx_mixture_avg_transport(kx_mech_t m, int n_states, int offsetT, int offset,
+                             double pressure, const double *d_state, double *d_viscosity,
+                             double *d_conductivity, double *d_rhoD, void *stream);
+
+/* Per-state-pressure variant (no reference counterpart: the reference takes one scalar
+ * p): d_p holds p [Pa] per state; if d_Dkm is non-NULL it also receives the mixture-averaged
+ * diffusion coefficients D_km [m^2/s] at that pressure. */
+int kx_mixture_avg_transport_pstate(kx_mech_t m, int n_states, int offsetT, int offset,
+                                    const double *d_p, const double *d_state,
+                                    double *d_viscosity, double *d_conductivity,
+                                    double *d_rhoD, double *d_Dkm, void *stream);
+
+/* kinetix::thermodynamicProps (kinetix.cpp:843-871) -> thermoCoeffs.okl: rho [kg/m^3],
+ * cp_k [J/kg/K] (n_species rows), rho*cp [J/m^3/K]. */
+int kx_thermodynamic_props(kx_mech_t m, int n_states, int offsetT, int offset,
+                           double pressure, const double *d_state, double *d_rho,
+                           double *d_cp, double *d_rhoCp, void *stream);
+
+/* Queries (kinetix.hpp:91-103; metadata kernels of benchmark/okl/mech.okl). */
+int kx_n_species(kx_mech_t m);
+int kx_n_active_species(kx_mech_t m);
+int kx_n_reactions(kx_mech_t m);
+/* Copies the k-th species name (NUL-terminated) into buf; returns its length or < 0. */
+int kx_species_name(kx_mech_t m, int k, char *buf, int buflen);
+int kx_species_index(kx_mech_t m, const char *name);       /* -1 if absent */
+/* molecularWeights(): W_k / Mbar_ref (kinetix.cpp:885-892); needs kx_build. */
+int kx_molecular_weights(kx_mech_t m, double *out);
+/* Molar masses W_k [kg/mol] (mech.h kinetix_molar_mass). */
+int kx_molar_masses(kx_mech_t m, double *out);
+double kx_ref_pressure(kx_mech_t m);
+double kx_ref_temperature(kx_mech_t m);
+double kx_ref_mean_molecular_weight(kx_mech_t m);
+
+/*
+ * Learned transport surrogate (no reference counterpart). A 55 -> 3x256 GELU -> 55 MLP
+ * (inputs: normalised T, ln p, Y_k; outputs: ln mu, ln lambda, ln D_km) with BF16 weights
+ * and FP32 accumulation on MFMA, trained by this library's own HIP kernels on states the
+ * plain BK2 kernel labels. See kinetix_mi_surrogate.h.
+ */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KINETIX_MI_H */

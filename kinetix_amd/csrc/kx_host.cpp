@@ -1,0 +1,310 @@
+// Host side of the C ABI declared in include/kinetix_mi.h.
+//
+// The reference's host API (benchmark/src/kinetix.cpp) JIT-builds OCCA kernels per
+// mechanism and keeps process-global state. Here each mechanism is an ahead-of-time
+// compiled gfx950 code object loaded with hipModuleLoad, held behind an integer handle;
+// launches go straight to hipModuleLaunchKernel on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/kinetix_mi.h"
+
+namespace {
+
+constexpr double kR = 1.380649e-23 * 6.02214076e23;   // kinetix.cpp:37
+constexpr int kBlock = 256;                            // launch bound of the generated kernels
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define KX_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(KX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+struct Mech {
+  int device = 0;
+  hipModule_t module = nullptr;
+  std::map<std::string, hipFunction_t> fn;
+  int n_species = 0, n_active = 0, n_reactions = 0;
+  std::vector<std::string> names;
+  std::vector<double> W;
+  std::string dir;
+  bool built = false;
+  double ref_p = 0, ref_T = 0, ref_Mbar = 0;
+  std::vector<double> ref_Y;
+};
+
+std::mutex g_mu;
+std::map<int, Mech*> g_mechs;
+int g_next = 1;
+
+Mech* get(kx_mech_t h)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_mechs.find(h);
+  return it == g_mechs.end() ? nullptr : it->second;
+}
+
+int read_meta(Mech* m, const std::string& path)
+{
+  std::ifstream f(path);
+  if (!f) return fail(KX_ERR_IO, "cannot open " + path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string key;
+    is >> key;
+    if (key == "n_species") is >> m->n_species;
+    else if (key == "n_active_species") is >> m->n_active;
+    else if (key == "n_reactions") is >> m->n_reactions;
+    else if (key == "species") { std::string s; while (is >> s) m->names.push_back(s); }
+    else if (key == "molar_mass") { double w; while (is >> w) m->W.push_back(w); }
+  }
+  if (m->n_species <= 0 || (int)m->names.size() != m->n_species || (int)m->W.size() != m->n_species)
+    return fail(KX_ERR_IO, "malformed " + path);
+  return KX_OK;
+}
+
+// Launch helper: packs the kernel arguments, 1-D grid of kBlock-lane workgroups.
+int launch(Mech* m, const char* name, int n_states, void* stream, void** args,
+           unsigned block = kBlock, unsigned states_per_block = kBlock, unsigned shmem = 0)
+{
+  auto it = m->fn.find(name);
+  if (it == m->fn.end() || !it->second)
+    return fail(KX_ERR_NO_KERNEL, std::string("kernel ") + name + " not in " + m->dir);
+  if (n_states == 0) return KX_OK;
+  KX_HIP(hipSetDevice(m->device));
+  const unsigned grid = (unsigned)((n_states + (long long)states_per_block - 1) / states_per_block);
+  KX_HIP(hipModuleLaunchKernel(it->second, grid, 1, 1, block, 1, 1, shmem,
+                               (hipStream_t)stream, args, nullptr));
+  return KX_OK;
+}
+
+int check_layout(Mech* m, int n, int offsetT, int offset, const void* a, const void* b)
+{
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!m->built) return fail(KX_ERR_NOT_BUILT, "kx_build has not been called");
+  if (n < 0) return fail(KX_ERR_INVALID_ARG, "n_states < 0");
+  if (n > 0 && (!a || !b)) return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  if (offsetT < n || offset < n)
+    return fail(KX_ERR_INVALID_ARG, "offsetT and offset must be >= n_states");
+  return KX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kx_last_error(void) { return g_err.c_str(); }
+
+int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
+{
+  if (!mech_dir || !out) return fail(KX_ERR_INVALID_ARG, "null argument");
+  Mech* m = new Mech();
+  m->dir = mech_dir;
+  m->device = device_id;
+  int rc = read_meta(m, m->dir + "/mech.txt");
+  if (rc) { delete m; return rc; }
+  hipError_t e = hipSetDevice(device_id);
+  if (e == hipSuccess) e = hipModuleLoad(&m->module, (m->dir + "/kernels.hsaco").c_str());
+  if (e != hipSuccess) {
+    delete m;
+    return fail(e == hipErrorFileNotFound ? KX_ERR_IO : KX_ERR_HIP,
+                std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
+  }
+  for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk2_fp64", "kx_bk2_fp64_pstate",
+                        "kx_thermo_fp64"}) {
+    hipFunction_t f = nullptr;
+    if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
+    else (void)hipGetLastError();
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  *out = g_next++;
+  g_mechs[*out] = m;
+  return KX_OK;
+}
+
+int kx_build(kx_mech_t h, double ref_pressure, double ref_temperature, const double* refY)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!refY || !(ref_pressure > 0) || !(ref_temperature > 0))
+    return fail(KX_ERR_INVALID_ARG, "reference state must be positive with mass fractions");
+  double sum = 0.;
+  m->ref_Y.assign(refY, refY + m->n_species);
+  for (int k = 0; k < m->n_species; ++k) sum += refY[k] / m->W[k];
+  if (!(sum > 0)) return fail(KX_ERR_INVALID_ARG, "reference mass fractions sum to zero");
+  m->ref_Mbar = 1. / sum;
+  m->ref_p = ref_pressure;
+  m->ref_T = ref_temperature;
+  m->built = true;
+  return KX_OK;
+}
+
+int kx_finalize(kx_mech_t h)
+{
+  Mech* m = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_mechs.find(h);
+    if (it == g_mechs.end()) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+    m = it->second;
+    g_mechs.erase(it);
+  }
+  if (m->module) { hipSetDevice(m->device); hipModuleUnload(m->module); }
+  delete m;
+  return KX_OK;
+}
+
+static int production_rates_impl(const char* kernel, kx_mech_t h, int n, int offsetT,
+                                 int offset, double pressure, const double* d_state,
+                                 double* d_rates, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_rates);
+  if (rc) return rc;
+  // kinetix.cpp:802-803
+  double pressure_R = pressure * m->ref_p / kR;
+  double pressure_ = pressure * m->ref_p;
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  const double* s = d_state;
+  double* r = d_rates;
+  void* args[] = {&n, &oT, &o, &pressure_R, &pressure_, &s, &r, &Tref};
+  return launch(m, kernel, n, stream, args);
+}
+
+int kx_production_rates(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                        const double* d_state, double* d_rates, void* stream)
+{
+  return production_rates_impl("kx_bk1_fp64", h, n, offsetT, offset, pressure, d_state, d_rates,
+                               stream);
+}
+
+int kx_production_rates_mixed(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                              const double* d_state, double* d_rates, void* stream)
+{
+  return production_rates_impl("kx_bk1_mixed", h, n, offsetT, offset, pressure, d_state, d_rates,
+                               stream);
+}
+
+static int transport_impl(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                          const double* d_p, const double* d_state, double* d_visc,
+                          double* d_cond, double* d_rhoD, double* d_Dkm, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_rhoD);
+  if (rc) return rc;
+  if (n > 0 && (!d_visc || !d_cond)) return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  const double* s = d_state;
+  if (d_Dkm) {
+    // per-state pressure, D_km output (conductivity, viscosity and rho*D do not depend on p)
+    const double* p = d_p;
+    void* args[] = {&n, &oT, &o, &p, &s, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
+    return launch(m, "kx_bk2_fp64_pstate", n, stream, args);
+  }
+  void* args[] = {&n, &oT, &o, &pressure, &s, &d_cond, &d_visc, &d_rhoD, &Tref};
+  return launch(m, "kx_bk2_fp64", n, stream, args);
+}
+
+int kx_mixture_avg_transport(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                             const double* d_state, double* d_visc, double* d_cond,
+                             double* d_rhoD, void* stream)
+{
+  return transport_impl(h, n, offsetT, offset, pressure, nullptr, d_state, d_visc, d_cond, d_rhoD,
+                        nullptr, stream);
+}
+
+int kx_mixture_avg_transport_pstate(kx_mech_t h, int n, int offsetT, int offset,
+                                    const double* d_p, const double* d_state, double* d_visc,
+                                    double* d_cond, double* d_rhoD, double* d_Dkm, void* stream)
+{
+  if (n > 0 && !d_p) return fail(KX_ERR_INVALID_ARG, "null pressure array");
+  return transport_impl(h, n, offsetT, offset, 1.0, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm,
+                        stream);
+}
+
+int kx_thermodynamic_props(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                           const double* d_state, double* d_rho, double* d_cp, double* d_rhoCp,
+                           void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_cp);
+  if (rc) return rc;
+  if (n > 0 && (!d_rho || !d_rhoCp)) return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  double pressure_R = pressure * m->ref_p / kR;   // kinetix.cpp:858
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  const double* s = d_state;
+  void* args[] = {&n, &oT, &o, &pressure_R, &s, &d_rho, &d_cp, &d_rhoCp, &Tref};
+  return launch(m, "kx_thermo_fp64", n, stream, args);
+}
+
+int kx_n_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_species : KX_ERR_NOT_INITIALIZED; }
+int kx_n_active_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_active : KX_ERR_NOT_INITIALIZED; }
+int kx_n_reactions(kx_mech_t h) { Mech* m = get(h); return m ? m->n_reactions : KX_ERR_NOT_INITIALIZED; }
+
+int kx_species_name(kx_mech_t h, int k, char* buf, int buflen)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (k < 0 || k >= m->n_species || !buf || buflen <= 0) return fail(KX_ERR_INVALID_ARG, "bad index");
+  const std::string& s = m->names[k];
+  if ((int)s.size() >= buflen) return fail(KX_ERR_INVALID_ARG, "buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+
+int kx_species_index(kx_mech_t h, const char* name)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!name) return -1;
+  for (int k = 0; k < m->n_species; ++k)
+    if (m->names[k] == name) return k;
+  return -1;
+}
+
+int kx_molecular_weights(kx_mech_t h, double* out)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!m->built) return fail(KX_ERR_NOT_BUILT, "kx_build has not been called");
+  if (!out) return fail(KX_ERR_INVALID_ARG, "null output");
+  for (int k = 0; k < m->n_species; ++k) out[k] = m->W[k] / m->ref_Mbar;
+  return KX_OK;
+}
+
+int kx_molar_masses(kx_mech_t h, double* out)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!out) return fail(KX_ERR_INVALID_ARG, "null output");
+  for (int k = 0; k < m->n_species; ++k) out[k] = m->W[k];
+  return KX_OK;
+}
+
+double kx_ref_pressure(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_p : NAN; }
+double kx_ref_temperature(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_T : NAN; }
+double kx_ref_mean_molecular_weight(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_Mbar : NAN; }
+
+}  // extern "C"

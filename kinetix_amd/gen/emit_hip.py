@@ -1,0 +1,542 @@
+"""gfx950 HIP emitter: writes mechanism-specialised CDNA4 kernels from the IR tables.
+
+One file per mechanism (``<out>/kernels.hip``), compiled ahead of time to a code object
+(``kernels.hsaco``) that the host library loads with hipModuleLoad. Kernels:
+
+  kx_bk1_fp64     species production rates + heat release rate, FP64
+                  (the reference's productionRates kernel around kinetix_species_rates and
+                  kinetix_enthalpy_RT: benchmark/okl/productionRates.okl:10-62,
+                  kinetix/core/reaction_rates.py:616-1333)
+  kx_bk2_fp64     mixture-averaged conductivity, viscosity (Wilke) and rho*D_km, FP64
+                  (benchmark/okl/transportProps.okl:11-47, kinetix/core/mix_transport.py:261-471)
+  kx_thermo_fp64  rho, cp_k, rho*cp (benchmark/okl/thermoCoeffs.okl:10-39)
+
+Design (MI355X-first, not a translation of the reference's rolled C):
+  * one state per lane, SoA loads/stores coalesce across the 64-wide wavefront;
+  * the mechanism is straight-line code: every table index is a compile-time constant, so
+    per-state arrays (concentrations, Gibbs exponentials, wdot) live in registers, not in
+    scratch, and no rate constant / q_r array is ever materialised;
+  * rate constants sharing (beta, E/R) share one exp; reactions are emitted grouped by it;
+  * k = A * exp(beta lnT - Ta/T) (A kept exact instead of exp(ln A + ...));
+  * Troe's 10^x is exp10(x).
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+from . import mechanism as M
+
+BLOCK = 256
+CFLOAT_MIN = '1e-300'
+
+
+# FP64 values that gfx9 VOP3 instructions encode as inline constants (no materialisation).
+_INLINE = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
+
+
+class ConstPool:
+    """Per-kernel table of FP64 constants, emitted as a __device__ array and read with
+    compile-time offsets. Keeping them in memory (scalar loads next to their use) instead
+    of as literals stops the compiler from hoisting ~10^3-10^4 64-bit immediates into SGPRs
+    for the whole kernel and spilling them."""
+
+    def __init__(self, name):
+        self.name = name
+        self.values = []
+        self.index = {}
+
+    def __call__(self, x):
+        x = float(x)
+        if x in _INLINE or math.isinf(x):
+            return _literal(x)
+        key = x.hex()
+        if key not in self.index:
+            self.index[key] = len(self.values)
+            self.values.append(x)
+        return f'{self.name}[{self.index[key]}]'
+
+    def declaration(self):
+        if not self.values:
+            return ''
+        body = ', '.join(_literal(v) for v in self.values)
+        return f'__device__ double {self.name}[{len(self.values)}] = {{{body}}};\n'
+
+
+_POOL = None
+
+
+def lit(x):
+    """FP64 constant: from the active kernel's pool if one is set, else a literal."""
+    return _POOL(x) if _POOL is not None else _literal(x)
+
+
+def _literal(x):
+    """Exact FP64 literal."""
+    x = float(x)
+    if x == 0.0:
+        return '0.0'
+    if math.isinf(x):
+        return '__builtin_inf()' if x > 0 else '(-__builtin_inf())'
+    r = repr(x)
+    if 'e' not in r and '.' not in r:
+        r += '.0'
+    return r
+
+
+class _Src:
+    def __init__(self):
+        self.lines = []
+
+    def __call__(self, s='', ind=1):
+        self.lines.append('  ' * ind + s if s else '')
+
+    def text(self):
+        return '\n'.join(self.lines)
+
+
+def _prod(var, nu):
+    """Product of concentrations with multiplicity, species order; '' if none."""
+    terms = []
+    for k, c in enumerate(nu):
+        terms += [f'{var}{k}'] * int(c)
+    return '*'.join(terms)
+
+
+# --------------------------------------------------------------------------------------
+# BK1
+# --------------------------------------------------------------------------------------
+
+class _RateEmitter:
+    """Emits the kinetics of one mechanism for one lane (FP64)."""
+
+    def __init__(self, t, info):
+        self.t = t
+        self.N = int(t['n_species'])
+        self.R = int(t['n_reactions'])
+        self.nu_f = t['nu_f']
+        self.nu_r = t['nu_r']
+        self.net = self.nu_r - self.nu_f
+        self.kind = t['kind']
+        self.rev = t['reversible'].astype(bool)
+        self.has_eff = t['has_eff'].astype(bool)
+        self.active = [k for k in range(self.N) if np.any(self.net[:, k] != 0)]
+        self.equations = info.get('equations') if info else None
+        self.sched_window = 1      # emit a scheduling barrier after every N-th reaction
+
+    # unique efficiency vectors -> eff variables
+    def efficiency_sets(self):
+        sets, idx = [], {}
+        for r in range(self.R):
+            if self.has_eff[r] and self.kind[r] != M.ELEMENTARY:
+                key = tuple(self.t['eff'][r])
+                if key not in idx:
+                    idx[key] = len(sets)
+                    sets.append(key)
+        return sets, idx
+
+    def arrh_key(self, beta, Ta):
+        beta, Ta = float(beta), float(Ta)
+        if Ta == 0.0 and beta in (0.0, -2.0, -1.0, 1.0, 2.0):
+            return None
+        return (beta, Ta)
+
+    def arrh_expr(self, A, beta, Ta, expvar):
+        beta, Ta = float(beta), float(Ta)
+        if Ta == 0.0:
+            if beta == 0.0:
+                return lit(A)
+            if beta == -2.0:
+                return f'{lit(A)} * rcpT2'
+            if beta == -1.0:
+                return f'{lit(A)} * rcpT'
+            if beta == 1.0:
+                return f'{lit(A)} * T'
+            if beta == 2.0:
+                return f'{lit(A)} * T2'
+        return f'{lit(A)} * {expvar}'
+
+    def exp_arg(self, beta, Ta):
+        beta, Ta = float(beta), float(Ta)
+        parts = []
+        if beta != 0.0:
+            parts.append(f'{lit(beta)} * lnT')
+        if Ta != 0.0:
+            parts.append(f'{lit(-Ta)} * rcpT')
+        return ' + '.join(parts)
+
+    def reaction_order(self):
+        """Group reactions by their first Arrhenius exponent key so shared exps stay local."""
+        t = self.t
+        first = {}
+        keys = []
+        for r in range(self.R):
+            k = self.arrh_key(t['beta'][r], t['Ta'][r])
+            keys.append(k)
+            if k is not None and k not in first:
+                first[k] = r
+        return sorted(range(self.R), key=lambda r: (first.get(keys[r], r), r))
+
+    def emit_species_rates(self, S, cvar='c', wvar='w', qsink=None):
+        """Emit the body of kinetix_species_rates for registers c{k} -> w{k} (+=).
+        qsink(r, src_expr_q) may override how q_r is consumed (mixed precision)."""
+        t, N = self.t, self.N
+        S('// ---- third-body concentrations')
+        S(f'const double Cm = ' + ' + '.join(f'{cvar}{k}' for k in range(N)) + ';')
+        sets, sidx = self.efficiency_sets()
+        for i, eff in enumerate(sets):
+            terms = ['Cm']
+            for k, e in enumerate(eff):
+                if e == 1.0:
+                    continue
+                if e == 2.0:
+                    terms.append(f'+ {cvar}{k}')
+                elif e == 0.0:
+                    terms.append(f'- {cvar}{k}')
+                else:
+                    terms.append(f'+ {lit(e - 1.0)} * {cvar}{k}')
+            S(f'const double eff{i} = ' + ' '.join(terms) + ';')
+
+        # Gibbs exponentials for species that appear in reversible reactions
+        need_g = sorted({k for r in range(self.R) if self.rev[r]
+                         for k in range(N) if self.net[r, k] != 0})
+        need_rg = sorted({k for r in range(self.R) if self.rev[r]
+                          for k in range(N) if self.net[r, k] < 0})
+        S('// ---- Gibbs exponentials exp(g_k/RT); NASA-7 piece chosen by T > T_mid without')
+        S('// branching (both pieces evaluated, result selected), so the code stays one block')
+        tmid = t['T_mid']
+        for tm in sorted({float(tmid[k]) for k in need_g}):
+            S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
+        for k in need_g:
+            S(f'double eg{k} = exp(hi{_tag(tmid[k])} ? ({self._gibbs(t["g_hi"][k])}) : ({self._gibbs(t["g_lo"][k])}));')
+        for k in need_rg:
+            S(f'const double rg{k} = 1.0 / eg{k};')
+        S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
+        S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
+
+        S('// ---- reactions')
+        seen = {}
+        for r in self.reaction_order():
+            pre = []
+            kinf = self._rate_const(pre, t['A'][r], t['beta'][r], t['Ta'][r], seen)
+            kind = self.kind[r]
+            k0 = None
+            if kind in (M.LINDEMANN, M.TROE, M.SRI):
+                k0 = self._rate_const(pre, t['A0'][r], t['beta0'][r], t['Ta0'][r], seen)
+            for line in pre:            # shared exps live at function scope
+                S(line)
+            S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
+            if kind == M.ELEMENTARY:
+                kf = kinf
+            elif kind == M.THREE_BODY:
+                kf = f'({kinf}) * {self._third(r, sidx, cvar)}'
+            elif kind == M.PLOG:
+                raise NotImplementedError('PLOG kernels are not emitted yet')
+            else:
+                S(f'  const double kinf = {kinf};')
+                S(f'  const double Pr = ({k0}) * {self._third(r, sidx, cvar)} / (kinf + {CFLOAT_MIN});')
+                if kind == M.LINDEMANN:
+                    S(f'  const double kf = kinf * Pr / (1.0 + Pr);')
+                elif kind == M.TROE:
+                    A_, rT3, rT1, T2 = t['troe'][r]
+                    term3 = f' + exp({lit(-T2)} * rcpT)' if T2 >= 0 else ''
+                    S(f'  const double logFc = log10({lit(1.0 - A_)} * exp({lit(-rT3)} * T) + '
+                      f'{lit(A_)} * exp({lit(-rT1)} * T){term3});')
+                    S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                    S(f'  const double tc = -0.4 - 0.67 * logFc + logPr;')
+                    S(f'  const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {CFLOAT_MIN});')
+                    S(f'  const double F = exp10(logFc / (1.0 + tf * tf));')
+                    S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
+                elif kind == M.SRI:
+                    a, b, rc, d, e = t['sri'][r]
+                    S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                    S(f'  const double F = {lit(d)} * pow(T, {lit(e)}) * pow({lit(a)} * exp({lit(-b)} * rcpT)'
+                      f' + exp({lit(-rc)} * T), 1.0 / (1.0 + logPr * logPr));')
+                    S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
+                kf = 'kf'
+            fwd = _prod(cvar, self.nu_f[r]) or '1.0'
+            if self.rev[r]:
+                rev = _prod(cvar, self.nu_r[r]) or '1.0'
+                S(f'  const double q = ({kf}) * ({fwd} - {self._kr(r)} * {rev});')
+            else:
+                S(f'  const double q = ({kf}) * ({fwd});')
+            if qsink is not None:
+                qsink(S, r)
+            else:
+                for k in range(N):
+                    nu = int(self.net[r, k])
+                    if nu == 1:
+                        S(f'  {wvar}{k} += q;')
+                    elif nu == -1:
+                        S(f'  {wvar}{k} -= q;')
+                    elif nu != 0:
+                        S(f'  {wvar}{k} += {lit(nu)} * q;')
+            S('}')
+            if self.sched_window and (r % self.sched_window == 0):
+                S('__builtin_amdgcn_sched_barrier(0);')
+
+    def _gibbs(self, a):
+        return (f'{lit(a[0])} * (1.0 - lnT) + {lit(a[1])} * T + {lit(a[2])} * T2 + '
+                f'{lit(a[3])} * T3 + {lit(a[4])} * T4 + {lit(a[5])} * rcpT - {lit(a[6])}')
+
+    def _rate_const(self, pre, A, beta, Ta, seen):
+        """Expression of A T^beta exp(-Ta/T); a shared exp is appended to pre on first use."""
+        key = self.arrh_key(beta, Ta)
+        if key is None:
+            return self.arrh_expr(A, beta, Ta, None)
+        if key not in seen:
+            name = f'ea{len(seen)}'
+            seen[key] = name
+            pre.append(f'const double {name} = exp({self.exp_arg(beta, Ta)});')
+        return self.arrh_expr(A, beta, Ta, seen[key])
+
+    def _third(self, r, sidx, cvar):
+        if self.has_eff[r]:
+            return f'eff{sidx[tuple(self.t["eff"][r])]}'
+        if self.t['collider'][r] >= 0:
+            return f'{cvar}{int(self.t["collider"][r])}'
+        return 'Cm'
+
+    def _kr(self, r):
+        terms = []
+        dnu = 0
+        for k in range(self.N):
+            nu = int(self.net[r, k])
+            dnu += nu
+            if nu < 0:
+                terms += [f'rg{k}'] * (-nu)
+            elif nu > 0:
+                terms += [f'eg{k}'] * nu
+        terms += ['C0' if dnu < 0 else 'rcpC0'] * abs(dnu)
+        return '(' + '*'.join(terms) + ')' if terms else '1.0'
+
+
+def _emit_bk1_fp64(t, info):
+    N = int(t['n_species'])
+    em = _RateEmitter(t, info)
+    S = _Src()
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_bk1_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('            const double pressure_R, const double pressure_,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double T = Tref * state[id];')
+    S('const double rcpT = 1.0 / T;')
+    S('const double lnT = log(T);')
+    S('const double T2 = T * T;')
+    S('const double T3 = T * T * T;')
+    S('const double T4 = T * T * T * T;')
+    S('const double rcpT2 = rcpT * rcpT;')
+    S('(void)pressure_;')
+    S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
+    rcpW = t['rcpW']
+    for k in range(N):
+        S(f'double c{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S('const double rcpMbar = ' + ' + '.join(f'c{k}' for k in range(N)) + ';')
+    S('const double rho = pressure_R * rcpT / rcpMbar;')
+    for k in range(N):
+        S(f'c{k} *= rho;')
+    for k in em.active:
+        S(f'double w{k} = 0.0;')
+    em.emit_species_rates(S)
+    S('// ---- mass production rates and heat release rate')
+    W = t['W']
+    for k in range(N):
+        val = f'{lit(W[k])} * w{k}' if k in em.active else '0.0'
+        S(f'rates[id + offsetT + {k} * offset] = {val};')
+    S('double sum_h = 0.0;')
+    _emit_h_RT_sum(S, t, em.active)
+    S(f'rates[id] = {lit(-float(t["R"]))} * T * sum_h;')
+    S('}', 0)
+    return S.text()
+
+
+def _emit_h_RT_sum(S, t, active):
+    tmid = t['T_mid']
+    for tm in sorted({float(tmid[k]) for k in active}):
+        S(f'const bool hh{_tag(tm)} = T > {lit(tm)};')
+    for k in active:
+        S(f'sum_h += w{k} * (hh{_tag(tmid[k])} ? ({_hpoly(t["h_hi"][k])}) : ({_hpoly(t["h_lo"][k])}));')
+
+
+def _tag(tm):
+    return str(int(round(float(tm) * 1000)))
+
+
+def _hpoly(a):
+    return (f'{lit(a[0])} + {lit(a[1])} * T + {lit(a[2])} * T2 + {lit(a[3])} * T3 + '
+            f'{lit(a[4])} * T4 + {lit(a[5])} * rcpT')
+
+
+# --------------------------------------------------------------------------------------
+# BK2
+# --------------------------------------------------------------------------------------
+
+def _poly4(c):
+    return (f'{lit(c[0])} + {lit(c[1])} * lnT + {lit(c[2])} * lnT2 + {lit(c[3])} * lnT3 + '
+            f'{lit(c[4])} * lnT4')
+
+
+def _emit_bk2_fp64(t, info, pstate=False):
+    """pstate=False: kx_bk2_fp64, scalar p (the reference's transport kernel).
+    pstate=True: kx_bk2_fp64_pstate, p [Pa] per state, also writes D_km."""
+    N = int(t['n_species'])
+    rcp_diff = bool(int(t.get('rcp_diff', 0)))
+    S = _Src()
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    if pstate:
+        S('kx_bk2_fp64_pstate(const int n_states, const long long offsetT, const long long offset,', 0)
+        S('            const double* __restrict__ p_state,', 0)
+    else:
+        S('kx_bk2_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
+        S('            const double pressure,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
+    S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
+    if pstate:
+        S('            double* __restrict__ Dkm, const double Tref)', 0)
+    else:
+        S('            const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
+    S('const double T = Tref * state[id];')
+    S('const double lnT = log(T);')
+    S('const double rcpT = 1.0 / T;')
+    S('const double sqrT = sqrt(T);')
+    S('const double lnT2 = lnT * lnT;')
+    S('const double lnT3 = lnT * lnT * lnT;')
+    S('const double lnT4 = lnT * lnT * lnT * lnT;')
+    rcpW, W = t['rcpW'], t['W']
+    for k in range(N):
+        S(f'double x{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S('const double rcpMbar = ' + ' + '.join(f'x{k}' for k in range(N)) + ';')
+    S('const double Mbar = 1.0 / rcpMbar;')
+    for k in range(N):
+        S(f'x{k} *= Mbar;')
+    # diffusion first: x and the N row sums are live
+    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj')
+    for k in range(N):
+        S(f'double s{k} = 0.0;')
+    d = t['diff']
+    for k in range(1, N):
+        S('__builtin_amdgcn_sched_barrier(0);')
+        for j in range(k):
+            if rcp_diff:
+                S(f'{{ const double rD = {_poly4(d[k, j])}; s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
+            else:
+                S(f'{{ const double rD = 1.0 / ({_poly4(d[k, j])}); s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
+    S('const double TsqrT = T * sqrT;')
+    S(f'const double rho = p * {lit(1.0 / float(t["R"]))} * rcpT * Mbar;')
+    S('const double pMbar = p * Mbar;')
+    for k in range(N):
+        S(f'{{ const double D = TsqrT * (Mbar - {lit(W[k])} * x{k}) / (pMbar * s{k});')
+        S(f'  rhoD[{k} * offset + id] = rho * D;' + (f' Dkm[{k} * offset + id] = D; }}' if pstate else ' }'))
+    # viscosity: Wilke, eta_k = poly, Phi^(1/2)_kj = C1 + C2 eta_k / eta_j
+    S('__builtin_amdgcn_sched_barrier(0);')
+    S('// ---- viscosity (Wilke)')
+    for k in range(N):
+        S(f'const double re{k} = 1.0 / ({_poly4(t["visc"][k])});')
+    S('double vis = 0.0;')
+    C1, C2 = t['C1'], t['C2']
+    for k in range(N):
+        S('__builtin_amdgcn_sched_barrier(0);')
+        S('{')
+        S(f'  const double e = {_poly4(t["visc"][k])};')
+        S('  double s = 0.0;')
+        for j in range(N):
+            S(f'  {{ const double f = {lit(C1[k, j])} + {lit(C2[k, j])} * e * re{j}; s += x{j} * f * f; }}')
+        S(f'  vis += x{k} * e * e / s;')
+        S('}')
+    S('__builtin_amdgcn_sched_barrier(0);')
+    S('viscosity[id] = sqrT * vis;')
+    S('// ---- conductivity')
+    S('double sum1 = 0.0, sum2 = 0.0;')
+    for k in range(N):
+        S(f'{{ const double l = {_poly4(t["cond"][k])}; sum1 += x{k} * l; sum2 += x{k} / l; }}')
+    S('conductivity[id] = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+    S('}', 0)
+    return S.text()
+
+
+# --------------------------------------------------------------------------------------
+# thermoCoeffs
+# --------------------------------------------------------------------------------------
+
+def _emit_thermo_fp64(t, info):
+    N = int(t['n_species'])
+    S = _Src()
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_thermo_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('               const double pressure_R, const double* __restrict__ state,', 0)
+    S('               double* __restrict__ rho_out, double* __restrict__ cp,', 0)
+    S('               double* __restrict__ rhoCp, const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double T = Tref * state[id];')
+    S('const double rcpT = 1.0 / T;')
+    S('const double T2 = T * T;')
+    S('const double T3 = T * T * T;')
+    S('const double T4 = T * T * T * T;')
+    rcpW = t['rcpW']
+    R = float(t['R'])
+    for k in range(N):
+        S(f'const double y{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S('const double rcpMbar = ' + ' + '.join(f'y{k}' for k in range(N)) + ';')
+    S('const double Mbar = 1.0 / rcpMbar;')
+    S('const double rho = pressure_R * rcpT * Mbar;')
+    S('rho_out[id] = rho;')
+    S('double mean = 0.0;')
+    tmid = t['T_mid']
+    for tm in sorted({float(x) for x in tmid}):
+        S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
+    for k in range(N):
+        lo, hi = t['cp_lo'][k], t['cp_hi'][k]
+        def cpoly(a):
+            return f'{lit(a[0])} + {lit(a[1])} * T + {lit(a[2])} * T2 + {lit(a[3])} * T3 + {lit(a[4])} * T4'
+        S(f'{{ const double c = hi{_tag(tmid[k])} ? ({cpoly(hi)}) : ({cpoly(lo)});')
+        S(f'  cp[{k} * offset + id] = c * {lit(R * rcpW[k])};')
+        S(f'  mean += c * y{k} * Mbar; }}')
+    S(f'rhoCp[id] = rho * (mean * {lit(R)} * rcpMbar);')
+    S('}', 0)
+    return S.text()
+
+
+HEADER = '''// GENERATED by kinetix_amd.gen.emit_hip for mechanism "{name}" -- do not edit.
+// {n_species} species ({n_active} active), {n_reactions} reactions. Target: gfx950 (CDNA4).
+#include <hip/hip_runtime.h>
+#include <math.h>
+'''
+
+
+def _with_pool(name, fn, *args):
+    global _POOL
+    _POOL = ConstPool(name)
+    try:
+        body = fn(*args)
+        return _POOL.declaration() + body
+    finally:
+        _POOL = None
+
+
+def generate(tables, info):
+    parts = [HEADER.format(name=info['name'], n_species=info['n_species'],
+                           n_active=info['n_active_species'], n_reactions=info['n_reactions'])]
+    parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info))
+    if 'cond' in tables:
+        parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False))
+        parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True))
+    parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
+    return '\n\n'.join(parts) + '\n'
+
+
+def write(out_dir, mech, tables):
+    from .tables import meta
+    src = generate(tables, meta(mech))
+    with open(os.path.join(out_dir, 'kernels.hip'), 'w') as fh:
+        fh.write(src)
