@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path: species production rates (BK1) + mixture-averaged transport
+(BK2) over GRI-3.0 states on one MI355X.
+
+One step = one BK1 launch + one BK2 launch over all N_STATES states, inputs resident in HBM
+(the reference's timing loop, benchmark/src/bk.cpp:694-776, without H2D/D2H). Prints ONE
+JSON line (driver contract) with the roofline of the dominant kernel, the CPU baseline
+(the reference's own FP64 routines, oracle/_ref, SERIAL flags, timed on this host's cores on
+a bounded sample), and the error of every output against the FP64 oracle.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--states N]
+                    [--bk1 fp64|mixed] [--bk2 fp64|surrogate] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'Mstates/s for rates & transport (GRI-3.0, 4M states) at stated err vs FP64'
+PEAK_FP64 = 78.6      # TFLOP/s, MI355X FP64 (vector = matrix)
+PEAK_FP32 = 157.3     # TFLOP/s
+PEAK_HBM = 8000.0     # GB/s
+P_ATM = 101325.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--states', type=int, default=4194304)
+    ap.add_argument('--mech', default='gri30')
+    ap.add_argument('--bk1', choices=('fp64', 'mixed'), default='fp64')
+    ap.add_argument('--bk2', choices=('fp64', 'surrogate'), default='fp64')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--cpu-seconds', type=float, default=10.0,
+                    help='target wall time of the bounded CPU-baseline sample')
+    ap.add_argument('--err-sample', type=int, default=4096)
+    ap.add_argument('--profile-only', action='store_true',
+                    help='for rocprofv3 --pmc: one device copy of a known size (HBM byte '
+                         'calibration), then one launch of each kernel; no timing, no JSON')
+    return ap.parse_args()
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(state, p, cores, target_s):
+    """Time the reference's own FP64 GRI routines (BK1 + BK2 per state) on this host."""
+    from oracle import pyoracle as po
+    kind = 'reference' if po.Reference.available('fast') else 'port'
+    n_all = state.shape[1]
+    if kind == 'reference':
+        ref = po.Reference('fast')
+
+        def run(s):
+            ref.production_rates(s, p, threads=cores)
+            ref.transport(s, 1.0, threads=cores)
+    else:
+        from kinetix_amd.gen import tables
+        _, t = tables.load(os.path.join(ROOT, 'kinetix_amd', 'mechs', 'gri30'))
+        orc = po.Oracle(t, fast=True)
+
+        def run(s):
+            orc.production_rates(s, p, threads=cores)
+            orc.transport(s, 1.0, threads=cores)
+    n = min(n_all, 4096 * cores)
+    s = np.ascontiguousarray(state[:, :n])
+    t0 = time.perf_counter(); run(s); dt = time.perf_counter() - t0
+    # scale the sample to ~target_s of wall time on `cores` threads
+    n = int(min(n_all, max(n, n * target_s / max(dt, 1e-6))))
+    s = np.ascontiguousarray(state[:, :n])
+    run(s[:, :min(n, 1024)])                     # warm
+    t0 = time.perf_counter(); run(s); dt = time.perf_counter() - t0
+    return {'value': round(n / dt / 1e6, 4), 'unit': 'Mstates/s', 'cores': cores, 'kind': kind,
+            'sample': f'{n} of the {n_all} states, BK1 + BK2 FP64, '
+                      f'{"reference GRI routines (SERIAL flags)" if kind == "reference" else "oracle port (-O3 -ffast-math)"}, '
+                      f'{cores} threads, {dt:.1f} s'}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f'bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); states '
+              f'shard across GPUs with zero exchange, so run one process per GPU instead.',
+              file=sys.stderr)
+        return 2
+    import torch
+    from kinetix_amd import states
+    from kinetix_amd.api import Kinetix
+    from kinetix_amd.gen import counts as cnt
+    from kinetix_amd.gen import tables
+
+    mdir = os.path.join(ROOT, 'kinetix_amd', 'mechs', a.mech)
+    info, t = tables.load(mdir)
+    counts = cnt.load(mdir)
+    kx = Kinetix(a.mech)
+    sp, W = kx.speciesNames(), kx.molar_masses
+    kx.build(P_ATM, 1.0, states.air_mass_fractions(sp, W))
+    N, n = kx.nSpecies(), a.states
+
+    t0 = time.time()
+    st, _ = states.sample(n, sp, W, T_range=(1000.0, 2500.0), seed=states.BASE_SEED + 2)
+    gen_s = time.time() - t0
+    dev = torch.device('cuda')
+    d_state = torch.from_numpy(st).to(dev).reshape(-1)
+    d_rates = torch.empty_like(d_state)
+    d_visc = torch.empty(n, dtype=torch.float64, device=dev)
+    d_cond = torch.empty_like(d_visc)
+    d_rhoD = torch.empty(N * n, dtype=torch.float64, device=dev)
+
+    bk1_name = 'kx_bk1_mixed' if a.bk1 == 'mixed' else 'kx_bk1_fp64'
+    bk2_name = 'kx_bk2_fp64'
+
+    def bk1():
+        kx.productionRates(n, n, n, 1.0, d_state, d_rates, mixed=(a.bk1 == 'mixed'))
+
+    def bk2():
+        kx.mixtureAvgTransportProps(n, n, n, 1.0, d_state, d_visc, d_cond, d_rhoD)
+
+    if a.profile_only:
+        # calibration: a plain copy moves exactly nbytes in and nbytes out
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev)      # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        bk1(); torch.cuda.synchronize()
+        bk2(); torch.cuda.synchronize()
+        print(f'profile-only: copy {src.numel() * 8} B, {bk1_name} and {bk2_name} over {n} states')
+        return 0
+
+    for _ in range(a.warmup):
+        bk1(); bk2()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(); bk1(); e1.record(); bk2(); e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    t_bk1 = np.array([e0.elapsed_time(e1) for e0, e1, _ in ev])     # ms
+    t_bk2 = np.array([e1.elapsed_time(e2) for _, e1, e2 in ev])
+    ms_step = wall * 1e3 / a.steps
+
+    # errors against the FP64 oracle on a sample of states
+    from oracle import pyoracle as po
+    orc = po.Oracle(t)
+    cores = host_cores()
+    idx = np.random.default_rng(7).choice(n, min(a.err_sample, n), replace=False)
+    rates = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
+    ref, scale = orc.production_rates(st[:, idx], P_ATM, scale=True, threads=cores)
+    c0, v0, r0, _ = orc.transport(st[:, idx], 1.0, threads=cores)
+    cond = d_cond[idx].cpu().numpy(); visc = d_visc[idx].cpu().numpy()
+    rhoD = d_rhoD.reshape(N, n)[:, idx].cpu().numpy()
+
+    def rel(x, y, s=None):
+        den = np.abs(y) if s is None else np.maximum(np.abs(y), s)
+        d = np.abs(x - y)
+        return float(np.max(np.where(d == 0, 0, d / np.where(den == 0, 1e-300, den))))
+
+    wref = ref[1:]
+    l2 = [float(np.linalg.norm(rates[1 + k] - wref[k]) / np.linalg.norm(wref[k]))
+          for k in range(N) if np.linalg.norm(wref[k]) > 0]
+    err = {'bk1_max_rel_cond_aware': rel(rates, ref, scale),
+           'bk1_max_rel_l2_per_species': max(l2),
+           'bk2_conductivity_max_rel': rel(cond, c0), 'bk2_viscosity_max_rel': rel(visc, v0),
+           'bk2_rhoD_max_rel': rel(rhoD, r0), 'sample_states': int(len(idx))}
+
+    def roof(name, ms):
+        c = counts[name]
+        fl = c['flops'] * n / (ms * 1e-3) / 1e12
+        gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
+        peak = PEAK_FP64
+        return {'bound': 'mfma', 'pipe': 'FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)',
+                'achieved': round(fl, 3), 'peak': peak, 'unit': 'TFLOP/s',
+                'frac': round(fl / peak, 4), 'traffic': None,
+                'flops_per_state': c['flops'], 'transcendentals_per_state': c['transcendentals'],
+                'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
+                'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
+
+    m1, m2 = float(t_bk1.mean()), float(t_bk2.mean())
+    kern = {bk1_name: roof(bk1_name, m1), bk2_name: roof(bk2_name, m2)}
+    dom = bk1_name if m1 >= m2 else bk2_name
+    roofline = dict(kern[dom]); roofline['kernel'] = dom
+    traffic_path = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if os.path.exists(traffic_path):
+        with open(traffic_path) as fh:
+            tr = json.load(fh)
+        key = f'{dom}@{n}'
+        if key in tr:
+            roofline['traffic'] = tr[key]
+
+    out = {
+        'metric': METRIC,
+        'value': round(n / (ms_step * 1e-3) / 1e6, 3),
+        'unit': 'Mstates/s',
+        'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+        'ms_per_step': round(ms_step, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64' if a.bk1 == 'fp64' else 'f32+f64',
+        'data': 'synthetic',
+        'config': {'workload': f'{a.mech} BK1 production rates + BK2 mixture-averaged transport, '
+                               f'{n} states, T~U[1000,2500] K, 1 atm, Y=0.5 Dirichlet(0.5)+0.5 CH4/air',
+                   'mechanism': a.mech, 'n_states': n, 'bk1': a.bk1, 'bk2': a.bk2,
+                   'n_species': N, 'n_reactions': kx.nReactions()},
+        'roofline': roofline,
+        'kernels': kern,
+        'bk1_Mstates_s': round(n / m1 / 1e3, 2), 'bk2_Mstates_s': round(n / m2 / 1e3, 2),
+        'bk1_GRXN_s': round(n * kx.nReactions() / m1 / 1e6, 2),
+        'bk2_GDOF_s': round(n * (N + 2) / m2 / 1e6, 2),
+        'err_vs_fp64_oracle': err,
+        'state_gen_s': round(gen_s, 2),
+    }
+    if not a.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(st, P_ATM, cores, a.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -30,6 +30,8 @@ def main(argv=None):
         fits = transport_fit.fit(mech, reciprocal_diffusivity=args.fit_rcpdiffcoeffs)
     tab = tables.build(mech, fits)
     tables.save(args.output, mech, tab)
+    from . import counts
+    counts.write(args.output, tab)
     if not args.tables_only:
         from . import emit_hip
         emit_hip.write(args.output, mech, tab)
