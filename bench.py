@@ -36,7 +36,9 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--states', type=int, default=4194304)
     ap.add_argument('--mech', default='gri30')
-    ap.add_argument('--bk1', choices=('fp64', 'mixed'), default='fp64')
+    ap.add_argument('--bk1', choices=('fp64', 'mixed', 'mixed_mfma'), default='mixed',
+                    help='headline BK1 kernel: fp64, mixed (FP32 rates, FP64 K_c/net/sum, '
+                         'VALU accumulation) or mixed_mfma (q.nu on FP32 MFMA)')
     ap.add_argument('--bk2', choices=('fp64', 'surrogate'), default='fp64')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=10.0,
@@ -120,11 +122,13 @@ def main():
     d_cond = torch.empty_like(d_visc)
     d_rhoD = torch.empty(N * n, dtype=torch.float64, device=dev)
 
-    bk1_name = 'kx_bk1_mixed' if a.bk1 == 'mixed' else 'kx_bk1_fp64'
+    BK1 = {'fp64': ('kx_bk1_fp64', False), 'mixed': ('kx_bk1_mixed_valu', 'valu'),
+           'mixed_mfma': ('kx_bk1_mixed', 'mfma')}
+    bk1_name, bk1_mode = BK1[a.bk1]
     bk2_name = 'kx_bk2_fp64'
 
-    def bk1():
-        kx.productionRates(n, n, n, 1.0, d_state, d_rates, mixed=(a.bk1 == 'mixed'))
+    def bk1(mode=None):
+        kx.productionRates(n, n, n, 1.0, d_state, d_rates, mixed=bk1_mode if mode is None else mode)
 
     def bk2():
         kx.mixtureAvgTransportProps(n, n, n, 1.0, d_state, d_visc, d_cond, d_rhoD)
@@ -155,13 +159,32 @@ def main():
     t_bk2 = np.array([e1.elapsed_time(e2) for _, e1, e2 in ev])
     ms_step = wall * 1e3 / a.steps
 
-    # errors against the FP64 oracle on a sample of states
+    # every BK1 variant, timed on the same inputs (outside the headline loop), and its error
     from oracle import pyoracle as po
     orc = po.Oracle(t)
     cores = host_cores()
     idx = np.random.default_rng(7).choice(n, min(a.err_sample, n), replace=False)
-    rates = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
     ref, scale = orc.production_rates(st[:, idx], P_ATM, scale=True, threads=cores)
+    variants = {}
+    for vname, (kname, mode) in BK1.items():
+        bk1(mode); torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            bk1(mode)
+        e1.record(); torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        r_ = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
+        l2v = [float(np.linalg.norm(r_[1 + k] - ref[1 + k]) / np.linalg.norm(ref[1 + k]))
+               for k in range(N) if np.linalg.norm(ref[1 + k]) > 0]
+        variants[vname] = {'kernel': kname, 'ms': round(ms, 4), 'Mstates_s': round(n / ms / 1e3, 2),
+                           'max_rel_l2_per_species_vs_fp64_oracle': max(l2v)}
+    for v in variants.values():
+        v['speedup_vs_fp64'] = round(variants['fp64']['ms'] / v['ms'], 3)
+    bk1(); torch.cuda.synchronize()               # leave the headline variant's output
+
+    # errors of the headline outputs against the FP64 oracle on a sample of states
+    rates = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
     c0, v0, r0, _ = orc.transport(st[:, idx], 1.0, threads=cores)
     cond = d_cond[idx].cpu().numpy(); visc = d_visc[idx].cpu().numpy()
     rhoD = d_rhoD.reshape(N, n)[:, idx].cpu().numpy()
@@ -180,7 +203,7 @@ def main():
            'bk2_rhoD_max_rel': rel(rhoD, r0), 'sample_states': int(len(idx))}
 
     def roof(name, ms):
-        c = counts[name]
+        c = counts[name if name in counts else 'kx_bk1_fp64']
         fl = c['flops'] * n / (ms * 1e-3) / 1e12
         gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
         peak = PEAK_FP64
@@ -211,6 +234,7 @@ def main():
         'ms_per_step': round(ms_step, 4),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f64' if a.bk1 == 'fp64' else 'f32+f64',
+        'bk1_variants': variants,
         'data': 'synthetic',
         'config': {'workload': f'{a.mech} BK1 production rates + BK2 mixture-averaged transport, '
                                f'{n} states, T~U[1000,2500] K, 1 atm, Y=0.5 Dirichlet(0.5)+0.5 CH4/air',

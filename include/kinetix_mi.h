@@ -71,6 +71,12 @@ int kx_production_rates_mixed(kx_mech_t m, int n_states, int offsetT, int offset
                               double pressure, const double *d_state, double *d_rates,
                               void *stream);
 
+/* Same precision split as kx_production_rates_mixed, but the stoichiometric accumulation
+ * q.nu runs on the FP64 vector ALUs instead of FP32 MFMA (no LDS staging). */
+int kx_production_rates_mixed_valu(kx_mech_t m, int n_states, int offsetT, int offset,
+                                   double pressure, const double *d_state, double *d_rates,
+                                   void *stream);
+
 /* kinetix::mixtureAvgTransportProps (kinetix.cpp:815-841) -> transportProps.okl.
  * Argument order follows the reference API: viscosity [Pa s], conductivity [W/m/K],
  * rhoD (n_species rows, rho*D_km [kg/m/s], at [k*offset + id]). pressure is passed
@@ -107,6 +113,11 @@ int kx_molar_masses(kx_mech_t m, double *out);
 double kx_ref_pressure(kx_mech_t m);
 double kx_ref_temperature(kx_mech_t m);
 double kx_ref_mean_molecular_weight(kx_mech_t m);
+
+/* Diagnostics (no reference counterpart): runs one wave of v_mfma_f32_16x16x4_f32 on
+ * closed-form 16x4 and 4x16 operands laid out as the generated mixed-precision kernels
+ * assume, and writes the 16x16 product row-major to d_out (256 floats). */
+int kx_selftest_mfma_layout(float *d_out, void *stream);
 
 /*
  * Learned transport surrogate (no reference counterpart). A 55 -> 3x256 GELU -> 55 MLP

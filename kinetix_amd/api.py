@@ -39,7 +39,8 @@ def lib():
         L.kx_init.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
         L.kx_build.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
         L.kx_finalize.argtypes = [C.c_int]
-        for fn in ('kx_production_rates', 'kx_production_rates_mixed'):
+        for fn in ('kx_production_rates', 'kx_production_rates_mixed',
+                   'kx_production_rates_mixed_valu'):
             getattr(L, fn).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
         L.kx_mixture_avg_transport.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -136,10 +137,14 @@ class Kinetix:
 
     # kinetix::productionRates
     def productionRates(self, n_states, offsetT, offset, pressure, o_state, o_rates, mixed=False):
+        """mixed: False (FP64), True or 'mfma' (mixed precision, q.nu on FP32 MFMA),
+        'valu' (mixed precision, q.nu on FP64 VALU)."""
         need = offsetT + (self.n_species - 1) * offset + n_states if n_states else 0
         _check_tensor('state', o_state, need)
         _check_tensor('rates', o_rates, need)
-        fn = lib().kx_production_rates_mixed if mixed else lib().kx_production_rates
+        fn = {False: lib().kx_production_rates, True: lib().kx_production_rates_mixed,
+              'mfma': lib().kx_production_rates_mixed,
+              'valu': lib().kx_production_rates_mixed_valu}[mixed]
         _check(fn(self.h, int(n_states), int(offsetT), int(offset), float(pressure), _ptr(o_state),
                   _ptr(o_rates), _stream(o_state)))
 

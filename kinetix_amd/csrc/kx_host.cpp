@@ -129,8 +129,8 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
     return fail(e == hipErrorFileNotFound ? KX_ERR_IO : KX_ERR_HIP,
                 std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
   }
-  for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk2_fp64", "kx_bk2_fp64_pstate",
-                        "kx_thermo_fp64"}) {
+  for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
+                        "kx_bk2_fp64_pstate", "kx_thermo_fp64"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
     else (void)hipGetLastError();
@@ -203,6 +203,13 @@ int kx_production_rates_mixed(kx_mech_t h, int n, int offsetT, int offset, doubl
 {
   return production_rates_impl("kx_bk1_mixed", h, n, offsetT, offset, pressure, d_state, d_rates,
                                stream);
+}
+
+int kx_production_rates_mixed_valu(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                                   const double* d_state, double* d_rates, void* stream)
+{
+  return production_rates_impl("kx_bk1_mixed_valu", h, n, offsetT, offset, pressure, d_state,
+                               d_rates, stream);
 }
 
 static int transport_impl(kx_mech_t h, int n, int offsetT, int offset, double pressure,

@@ -128,7 +128,11 @@ def thermo(t):
 
 
 def all_counts(t):
-    out = {'kx_bk1_fp64': bk1(t), 'kx_bk1_mixed': bk1(t), 'kx_thermo_fp64': thermo(t)}
+    from . import emit_mixed
+    mixed = dict(bk1(t))
+    mixed.update(emit_mixed.mfma_per_state(t))
+    out = {'kx_bk1_fp64': bk1(t), 'kx_bk1_mixed': mixed, 'kx_bk1_mixed_valu': bk1(t),
+           'kx_thermo_fp64': thermo(t)}
     if 'cond' in t:
         out['kx_bk2_fp64'] = bk2(t)
         out['kx_bk2_fp64_pstate'] = bk2(t, pstate=True)

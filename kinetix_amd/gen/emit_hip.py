@@ -43,10 +43,12 @@ class ConstPool:
     of as literals stops the compiler from hoisting ~10^3-10^4 64-bit immediates into SGPRs
     for the whole kernel and spilling them."""
 
-    def __init__(self, name):
+    def __init__(self, name, lds=False):
         self.name = name
         self.values = []
         self.index = {}
+        self.lds = lds            # stage the table in LDS once per workgroup
+        self.ref = f'{name}_s' if lds else name
 
     def __call__(self, x):
         x = float(x)
@@ -56,7 +58,16 @@ class ConstPool:
         if key not in self.index:
             self.index[key] = len(self.values)
             self.values.append(x)
-        return f'{self.name}[{self.index[key]}]'
+        return f'{self.ref}[{self.index[key]}]'
+
+    def prologue(self, block):
+        """LDS staging code placed at the top of the kernel body (before any early exit)."""
+        if not self.lds or not self.values:
+            return ''
+        n = len(self.values)
+        return (f'  __shared__ double {self.ref}[{n}];\n'
+                f'  for (int i_ = threadIdx.x; i_ < {n}; i_ += {block}) {self.ref}[i_] = {self.name}[i_];\n'
+                f'  __syncthreads();\n')
 
     def declaration(self):
         if not self.values:
@@ -179,9 +190,8 @@ class _RateEmitter:
                 first[k] = r
         return sorted(range(self.R), key=lambda r: (first.get(keys[r], r), r))
 
-    def emit_species_rates(self, S, cvar='c', wvar='w', qsink=None):
-        """Emit the body of kinetix_species_rates for registers c{k} -> w{k} (+=).
-        qsink(r, src_expr_q) may override how q_r is consumed (mixed precision)."""
+    def emit_prelude(self, S, cvar='c'):
+        """Third-body sums, Gibbs exponentials (FP64) and C0; returns the eff-set index."""
         t, N = self.t, self.N
         S('// ---- third-body concentrations')
         S(f'const double Cm = ' + ' + '.join(f'{cvar}{k}' for k in range(N)) + ';')
@@ -198,8 +208,6 @@ class _RateEmitter:
                 else:
                     terms.append(f'+ {lit(e - 1.0)} * {cvar}{k}')
             S(f'const double eff{i} = ' + ' '.join(terms) + ';')
-
-        # Gibbs exponentials for species that appear in reversible reactions
         need_g = sorted({k for r in range(self.R) if self.rev[r]
                          for k in range(N) if self.net[r, k] != 0})
         need_rg = sorted({k for r in range(self.R) if self.rev[r]
@@ -215,73 +223,249 @@ class _RateEmitter:
             S(f'const double rg{k} = 1.0 / eg{k};')
         S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
         S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
+        return sidx
 
+    def emit_reaction(self, S, r, sidx, seen, cvar='c', mixed=False):
+        """Emit `double q<r>` = net rate of progress of reaction r (FP64).
+
+        mixed=False: every rate coefficient in FP64.
+        mixed=True:  Arrhenius exponentials through the FP32 hardware exp2 (FP64 argument,
+                     exact power-of-two scaling) and falloff corrections in FP32; K_c and the
+                     forward-minus-reverse subtraction stay FP64."""
+        t, N = self.t, self.N
+        pre = []
+        kinf = self._rate_const(pre, t['A'][r], t['beta'][r], t['Ta'][r], seen, mixed)
+        kind = self.kind[r]
+        k0 = None
+        if kind in (M.LINDEMANN, M.TROE, M.SRI):
+            k0 = self._rate_const(pre, t['A0'][r], t['beta0'][r], t['Ta0'][r], seen, mixed)
+        for line in pre:            # shared exps live at function scope
+            S(line)
+        S(f'double q{r};')
+        S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
+        if kind == M.ELEMENTARY:
+            kf = kinf
+        elif kind == M.THREE_BODY:
+            kf = f'({kinf}) * {self._third(r, sidx, cvar)}'
+        elif kind == M.PLOG:
+            raise NotImplementedError('PLOG kernels are not emitted yet')
+        elif not mixed:
+            S(f'  const double kinf = {kinf};')
+            S(f'  const double Pr = ({k0}) * {self._third(r, sidx, cvar)} / (kinf + {CFLOAT_MIN});')
+            if kind == M.LINDEMANN:
+                S(f'  const double kf = kinf * Pr / (1.0 + Pr);')
+            elif kind == M.TROE:
+                A_, rT3, rT1, T2 = t['troe'][r]
+                term3 = f' + exp({lit(-T2)} * rcpT)' if T2 >= 0 else ''
+                S(f'  const double logFc = log10({lit(1.0 - A_)} * exp({lit(-rT3)} * T) + '
+                  f'{lit(A_)} * exp({lit(-rT1)} * T){term3});')
+                S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                S(f'  const double tc = -0.4 - 0.67 * logFc + logPr;')
+                S(f'  const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {CFLOAT_MIN});')
+                S(f'  const double F = exp10(logFc / (1.0 + tf * tf));')
+                S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
+            elif kind == M.SRI:
+                a, b, rc, d, e = t['sri'][r]
+                S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                S(f'  const double F = {lit(d)} * pow(T, {lit(e)}) * pow({lit(a)} * exp({lit(-b)} * rcpT)'
+                  f' + exp({lit(-rc)} * T), 1.0 / (1.0 + logPr * logPr));')
+                S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
+            kf = 'kf'
+        else:
+            # FP32 falloff correction; k_inf and k_0 arrive FP64-scaled
+            S(f'  const double kinf = {kinf};')
+            S(f'  const float Pr = (float)(({k0}) * {self._third(r, sidx, cvar)}) * '
+              f'__builtin_amdgcn_rcpf((float)kinf + 1e-37f);')
+            S('  const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
+            if kind == M.LINDEMANN:
+                S('  const double kf = kinf * (double)Pr1;')
+            elif kind == M.TROE:
+                A_, rT3, rT1, T2 = t['troe'][r]
+                L2E = 1.4426950408889634
+                term3 = (f' + __builtin_amdgcn_exp2f({float(-T2 * L2E)!r}f * rcpTf)' if T2 >= 0 else '')
+                S(f'  const float logFc = {0.30102999566398120!r}f * __builtin_amdgcn_logf('
+                  f'{float(1.0 - A_)!r}f * __builtin_amdgcn_exp2f({float(-rT3 * L2E)!r}f * Tf) + '
+                  f'{float(A_)!r}f * __builtin_amdgcn_exp2f({float(-rT1 * L2E)!r}f * Tf){term3});')
+                S(f'  const float logPr = {0.30102999566398120!r}f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+                S('  const float tc = -0.4f - 0.67f * logFc + logPr;')
+                S('  const float tf = tc * __builtin_amdgcn_rcpf(0.75f - 1.27f * logFc - 0.14f * tc + 1e-37f);')
+                S(f'  const float F = __builtin_amdgcn_exp2f({3.3219280948873622!r}f * logFc * '
+                  f'__builtin_amdgcn_rcpf(1.0f + tf * tf));')
+                S('  const double kf = kinf * (double)(Pr1 * F);')
+            elif kind == M.SRI:
+                a, b, rc, d, e = t['sri'][r]
+                S(f'  const float logPr = {0.30102999566398120!r}f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+                S(f'  const float F = {float(d)!r}f * powf(Tf, {float(e)!r}f) * powf({float(a)!r}f * '
+                  f'expf({float(-b)!r}f * rcpTf) + expf({float(-rc)!r}f * Tf), '
+                  f'__builtin_amdgcn_rcpf(1.0f + logPr * logPr));')
+                S('  const double kf = kinf * (double)(Pr1 * F);')
+            kf = 'kf'
+        fwd = _prod(cvar, self.nu_f[r]) or '1.0'
+        if self.rev[r]:
+            rev = _prod(cvar, self.nu_r[r]) or '1.0'
+            S(f'  q{r} = ({kf}) * ({fwd} - {self._kr(r)} * {rev});')
+        else:
+            S(f'  q{r} = ({kf}) * ({fwd});')
+        S('}')
+
+    # ---- "w-form": q_r = [k_f exp(-sum_reactants g)] (prod_r w - C0^-dnu prod_p w), w = C exp(g)
+    def emit_prelude_w(self, S, cvar='c'):
+        """Third-body sums from C, then gl_k = g_k log2(e) and w_k = C_k exp(g_k) (FP64).
+        Concentrations are dead afterwards. Returns the eff-set index."""
+        t, N = self.t, self.N
+        S('// ---- third-body concentrations')
+        S(f'const double Cm = ' + ' + '.join(f'{cvar}{k}' for k in range(N)) + ';')
+        sets, sidx = self.efficiency_sets()
+        for i, eff in enumerate(sets):
+            terms = ['Cm']
+            for k, e in enumerate(eff):
+                if e == 1.0:
+                    continue
+                if e == 2.0:
+                    terms.append(f'+ {cvar}{k}')
+                elif e == 0.0:
+                    terms.append(f'- {cvar}{k}')
+                else:
+                    terms.append(f'+ {lit(e - 1.0)} * {cvar}{k}')
+            S(f'const double eff{i} = ' + ' '.join(terms) + ';')
+        colliders = sorted({int(t['collider'][r]) for r in range(self.R)
+                            if t['collider'][r] >= 0 and not self.has_eff[r]})
+        for k in colliders:
+            S(f'const double M{k} = {cvar}{k};')
+        used = sorted({k for r in range(self.R) for k in range(N)
+                       if self.nu_f[r, k] or (self.rev[r] and self.nu_r[r, k])})
+        S('// ---- Gibbs energies g_k (FP64): gl = g log2(e) for the rate exponents, w = C exp(g)')
+        tmid = t['T_mid']
+        for tm in sorted({float(tmid[k]) for k in used}):
+            S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
+        L2E = 1.4426950408889634
+        for k in used:
+            S(f'const double g{k} = hi{_tag(tmid[k])} ? ({self._gibbs(t["g_hi"][k])}) : ({self._gibbs(t["g_lo"][k])});')
+            S(f'const double gl{k} = g{k} * {lit(L2E)};')
+            S(f'const double w{k} = {cvar}{k} * exp(g{k});')
+        S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
+        S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
+        return sidx
+
+    def _third_w(self, r, sidx):
+        if self.has_eff[r]:
+            return f'eff{sidx[tuple(self.t["eff"][r])]}'
+        if self.t['collider'][r] >= 0:
+            return f'M{int(self.t["collider"][r])}'
+        return 'Cm'
+
+    def _log2_rate(self, A, beta, Ta, extra=''):
+        """FP64 expression of log2(A T^beta exp(-Ta/T)) (+ extra)."""
+        L2E = 1.4426950408889634
+        parts = [lit(math.log2(A))] if A != 1.0 else []
+        if float(beta) != 0.0:
+            parts.append(f'{lit(beta)} * log2T')
+        if float(Ta) != 0.0:
+            parts.append(f'{lit(-float(Ta) * L2E)} * rcpT')
+        if not parts:
+            parts = ['0.0']
+        return ' + '.join(parts) + extra
+
+    def emit_reaction_w(self, S, r, sidx, pow2='kx_pow2', fp64_falloff=False):
+        """Emit `double q<r>` in the w-form. The scale factor k_f exp(-sum g_r) comes from
+        one pow2() of an FP64 exponent (the Gibbs sum folded into the Arrhenius exponent);
+        falloff corrections are FP32; the bracket difference is FP64."""
+        t, N = self.t, self.N
+        kind = self.kind[r]
+        gsum = ''.join(f' - gl{k}' * int(self.nu_f[r, k]) for k in range(N) if self.nu_f[r, k])
+        S(f'double q{r};')
+        S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
+        if kind == M.PLOG:
+            raise NotImplementedError('PLOG kernels are not emitted yet')
+        S(f'  double F = {pow2}({self._log2_rate(t["A"][r], t["beta"][r], t["Ta"][r], gsum)});')
+        if kind == M.THREE_BODY:
+            S(f'  F *= {self._third_w(r, sidx)};')
+        elif kind in (M.LINDEMANN, M.TROE, M.SRI) and fp64_falloff:
+            ratio = math.log2(t['A0'][r] / t['A'][r])
+            db = float(t['beta0'][r]) - float(t['beta'][r])
+            dT = float(t['Ta0'][r]) - float(t['Ta'][r])
+            S(f'  const double Pr = {self._third_w(r, sidx)} * {pow2}({self._log2_rate(2.0 ** ratio, db, dT)});')
+            S('  const double Pr1 = Pr / (1.0 + Pr);')
+            if kind == M.LINDEMANN:
+                S('  F *= Pr1;')
+            elif kind == M.TROE:
+                A_, rT3, rT1, T2 = t['troe'][r]
+                term3 = f' + exp({lit(-T2)} * rcpT)' if T2 >= 0 else ''
+                S(f'  const double logFc = log10({lit(1.0 - A_)} * exp({lit(-rT3)} * T) + '
+                  f'{lit(A_)} * exp({lit(-rT1)} * T){term3});')
+                S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                S(f'  const double tc = -0.4 - 0.67 * logFc + logPr;')
+                S(f'  const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {CFLOAT_MIN});')
+                S(f'  F *= Pr1 * exp10(logFc / (1.0 + tf * tf));')
+            else:
+                a, b, rc, d, e = t['sri'][r]
+                S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
+                S(f'  F *= Pr1 * {lit(d)} * pow(T, {lit(e)}) * pow({lit(a)} * exp({lit(-b)} * rcpT)'
+                  f' + exp({lit(-rc)} * T), 1.0 / (1.0 + logPr * logPr));')
+        elif kind in (M.LINDEMANN, M.TROE, M.SRI):
+            ratio = math.log2(t['A0'][r] / t['A'][r])
+            db = float(t['beta0'][r]) - float(t['beta'][r])
+            dT = float(t['Ta0'][r]) - float(t['Ta'][r])
+            S(f'  const float Pr = (float)({self._third_w(r, sidx)} * '
+              f'{pow2}({self._log2_rate(2.0 ** ratio, db, dT)}));')
+            S('  const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
+            if kind == M.LINDEMANN:
+                S('  F *= (double)Pr1;')
+            elif kind == M.TROE:
+                A_, rT3, rT1, T2 = t['troe'][r]
+                L2E = 1.4426950408889634
+                term3 = (f' + __builtin_amdgcn_exp2f({float(-T2 * L2E)!r}f * rcpTf)' if T2 >= 0 else '')
+                S(f'  const float logFc = {0.30102999566398120!r}f * __builtin_amdgcn_logf('
+                  f'{float(1.0 - A_)!r}f * __builtin_amdgcn_exp2f({float(-rT3 * L2E)!r}f * Tf) + '
+                  f'{float(A_)!r}f * __builtin_amdgcn_exp2f({float(-rT1 * L2E)!r}f * Tf){term3});')
+                S(f'  const float logPr = {0.30102999566398120!r}f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+                S('  const float tc = -0.4f - 0.67f * logFc + logPr;')
+                S('  const float tf = tc * __builtin_amdgcn_rcpf(0.75f - 1.27f * logFc - 0.14f * tc + 1e-37f);')
+                S(f'  const float Fc = __builtin_amdgcn_exp2f({3.3219280948873622!r}f * logFc * '
+                  f'__builtin_amdgcn_rcpf(1.0f + tf * tf));')
+                S('  F *= (double)(Pr1 * Fc);')
+            else:
+                a, b, rc, d, e = t['sri'][r]
+                S(f'  const float logPr = {0.30102999566398120!r}f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+                S(f'  const float Fs = {float(d)!r}f * powf(Tf, {float(e)!r}f) * powf({float(a)!r}f * '
+                  f'expf({float(-b)!r}f * rcpTf) + expf({float(-rc)!r}f * Tf), '
+                  f'__builtin_amdgcn_rcpf(1.0f + logPr * logPr));')
+                S('  F *= (double)(Pr1 * Fs);')
+        fwd = _prod('w', self.nu_f[r]) or '1.0'
+        if self.rev[r]:
+            dnu = int(self.net[r].sum())
+            rev = _prod('w', self.nu_r[r]) or '1.0'
+            c0 = '*'.join(['C0' if dnu < 0 else 'rcpC0'] * abs(dnu))
+            rev_term = f'({c0}) * {rev}' if c0 else rev
+            S(f'  q{r} = F * ({fwd} - {rev_term});')
+        else:
+            S(f'  q{r} = F * ({fwd});')
+        S('}')
+
+    def emit_species_rates(self, S, cvar='c', wvar='w'):
+        """FP64 kinetix_species_rates body: registers c{k} -> w{k} (+=)."""
+        N = self.N
+        sidx = self.emit_prelude(S, cvar)
         S('// ---- reactions')
         seen = {}
-        for r in self.reaction_order():
-            pre = []
-            kinf = self._rate_const(pre, t['A'][r], t['beta'][r], t['Ta'][r], seen)
-            kind = self.kind[r]
-            k0 = None
-            if kind in (M.LINDEMANN, M.TROE, M.SRI):
-                k0 = self._rate_const(pre, t['A0'][r], t['beta0'][r], t['Ta0'][r], seen)
-            for line in pre:            # shared exps live at function scope
-                S(line)
-            S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
-            if kind == M.ELEMENTARY:
-                kf = kinf
-            elif kind == M.THREE_BODY:
-                kf = f'({kinf}) * {self._third(r, sidx, cvar)}'
-            elif kind == M.PLOG:
-                raise NotImplementedError('PLOG kernels are not emitted yet')
-            else:
-                S(f'  const double kinf = {kinf};')
-                S(f'  const double Pr = ({k0}) * {self._third(r, sidx, cvar)} / (kinf + {CFLOAT_MIN});')
-                if kind == M.LINDEMANN:
-                    S(f'  const double kf = kinf * Pr / (1.0 + Pr);')
-                elif kind == M.TROE:
-                    A_, rT3, rT1, T2 = t['troe'][r]
-                    term3 = f' + exp({lit(-T2)} * rcpT)' if T2 >= 0 else ''
-                    S(f'  const double logFc = log10({lit(1.0 - A_)} * exp({lit(-rT3)} * T) + '
-                      f'{lit(A_)} * exp({lit(-rT1)} * T){term3});')
-                    S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
-                    S(f'  const double tc = -0.4 - 0.67 * logFc + logPr;')
-                    S(f'  const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {CFLOAT_MIN});')
-                    S(f'  const double F = exp10(logFc / (1.0 + tf * tf));')
-                    S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
-                elif kind == M.SRI:
-                    a, b, rc, d, e = t['sri'][r]
-                    S(f'  const double logPr = log10(Pr + {CFLOAT_MIN});')
-                    S(f'  const double F = {lit(d)} * pow(T, {lit(e)}) * pow({lit(a)} * exp({lit(-b)} * rcpT)'
-                      f' + exp({lit(-rc)} * T), 1.0 / (1.0 + logPr * logPr));')
-                    S(f'  const double kf = kinf * (Pr / (1.0 + Pr)) * F;')
-                kf = 'kf'
-            fwd = _prod(cvar, self.nu_f[r]) or '1.0'
-            if self.rev[r]:
-                rev = _prod(cvar, self.nu_r[r]) or '1.0'
-                S(f'  const double q = ({kf}) * ({fwd} - {self._kr(r)} * {rev});')
-            else:
-                S(f'  const double q = ({kf}) * ({fwd});')
-            if qsink is not None:
-                qsink(S, r)
-            else:
-                for k in range(N):
-                    nu = int(self.net[r, k])
-                    if nu == 1:
-                        S(f'  {wvar}{k} += q;')
-                    elif nu == -1:
-                        S(f'  {wvar}{k} -= q;')
-                    elif nu != 0:
-                        S(f'  {wvar}{k} += {lit(nu)} * q;')
-            S('}')
-            if self.sched_window and (r % self.sched_window == 0):
+        for i, r in enumerate(self.reaction_order()):
+            self.emit_reaction(S, r, sidx, seen, cvar)
+            for k in range(N):
+                nu = int(self.net[r, k])
+                if nu == 1:
+                    S(f'{wvar}{k} += q{r};')
+                elif nu == -1:
+                    S(f'{wvar}{k} -= q{r};')
+                elif nu != 0:
+                    S(f'{wvar}{k} += {lit(nu)} * q{r};')
+            if self.sched_window and (i % self.sched_window == 0):
                 S('__builtin_amdgcn_sched_barrier(0);')
 
     def _gibbs(self, a):
         return (f'{lit(a[0])} * (1.0 - lnT) + {lit(a[1])} * T + {lit(a[2])} * T2 + '
                 f'{lit(a[3])} * T3 + {lit(a[4])} * T4 + {lit(a[5])} * rcpT - {lit(a[6])}')
 
-    def _rate_const(self, pre, A, beta, Ta, seen):
+    def _rate_const(self, pre, A, beta, Ta, seen, mixed=False):
         """Expression of A T^beta exp(-Ta/T); a shared exp is appended to pre on first use."""
         key = self.arrh_key(beta, Ta)
         if key is None:
@@ -289,7 +473,17 @@ class _RateEmitter:
         if key not in seen:
             name = f'ea{len(seen)}'
             seen[key] = name
-            pre.append(f'const double {name} = exp({self.exp_arg(beta, Ta)});')
+            if mixed:
+                b, Tk = key
+                L2E = 1.4426950408889634
+                parts = []
+                if b != 0.0:
+                    parts.append(f'{lit(b)} * log2T')
+                if Tk != 0.0:
+                    parts.append(f'{lit(-Tk * L2E)} * rcpT')
+                pre.append(f'const double {name} = kx_pow2({" + ".join(parts)});')
+            else:
+                pre.append(f'const double {name} = exp({self.exp_arg(beta, Ta)});')
         return self.arrh_expr(A, beta, Ta, seen[key])
 
     def _third(self, r, sidx, cvar):
@@ -313,15 +507,30 @@ class _RateEmitter:
         return '(' + '*'.join(terms) + ')' if terms else '1.0'
 
 
-def _emit_bk1_fp64(t, info):
+def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64'):
+    """variant 'fp64': plain FP64 kernel. variant 'mixed_valu': the mixed-precision rate
+    coefficients (FP32 exp2 with FP64 scaling, FP32 falloff) with FP64 VALU accumulation of
+    wdot instead of MFMA (w-form only)."""
     N = int(t['n_species'])
     em = _RateEmitter(t, info)
+    em.sched_window = sched_window
+    mixed = variant == 'mixed_valu'
+    if mixed:
+        wform = True
     S = _Src()
+    if mixed:
+        S('__device__ __forceinline__ double kx_pow2v(double y)', 0)
+        S('{', 0)
+        S('const double n = __builtin_rint(y);')
+        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
+        S('return __builtin_ldexp((double)m, (int)n);')
+        S('}', 0)
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
-    S('kx_bk1_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
+    S(f'kx_bk1_{variant}(const int n_states, const long long offsetT, const long long offset,', 0)
     S('            const double pressure_R, const double pressure_,', 0)
     S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
     S('{', 0)
+    S(POOL_MARK)
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
     S('const double T = Tref * state[id];')
@@ -331,7 +540,10 @@ def _emit_bk1_fp64(t, info):
     S('const double T3 = T * T * T;')
     S('const double T4 = T * T * T * T;')
     S('const double rcpT2 = rcpT * rcpT;')
-    S('(void)pressure_;')
+    S(f'const double log2T = lnT * {lit(1.4426950408889634)};')
+    S('const float Tf = (float)T;')
+    S('const float rcpTf = (float)rcpT;')
+    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf;')
     S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
     rcpW = t['rcpW']
     for k in range(N):
@@ -341,26 +553,44 @@ def _emit_bk1_fp64(t, info):
     for k in range(N):
         S(f'c{k} *= rho;')
     for k in em.active:
-        S(f'double w{k} = 0.0;')
-    em.emit_species_rates(S)
+        S(f'double wd{k} = 0.0;')
+    if wform:
+        sidx = em.emit_prelude_w(S, 'c')
+        for i, r in enumerate(em.reaction_order()):
+            if mixed:
+                em.emit_reaction_w(S, r, sidx, pow2='kx_pow2v', fp64_falloff=False)
+            else:
+                em.emit_reaction_w(S, r, sidx, pow2='exp2', fp64_falloff=True)
+            for k in range(N):
+                nu = int(em.net[r, k])
+                if nu == 1:
+                    S(f'wd{k} += q{r};')
+                elif nu == -1:
+                    S(f'wd{k} -= q{r};')
+                elif nu != 0:
+                    S(f'wd{k} += {lit(nu)} * q{r};')
+            if sched_window and (i % sched_window == 0):
+                S('__builtin_amdgcn_sched_barrier(0);')
+    else:
+        em.emit_species_rates(S, wvar='wd')
     S('// ---- mass production rates and heat release rate')
     W = t['W']
     for k in range(N):
-        val = f'{lit(W[k])} * w{k}' if k in em.active else '0.0'
+        val = f'{lit(W[k])} * wd{k}' if k in em.active else '0.0'
         S(f'rates[id + offsetT + {k} * offset] = {val};')
     S('double sum_h = 0.0;')
-    _emit_h_RT_sum(S, t, em.active)
+    _emit_h_RT_sum(S, t, em.active, wvar='wd')
     S(f'rates[id] = {lit(-float(t["R"]))} * T * sum_h;')
     S('}', 0)
     return S.text()
 
 
-def _emit_h_RT_sum(S, t, active):
+def _emit_h_RT_sum(S, t, active, wvar='w'):
     tmid = t['T_mid']
     for tm in sorted({float(tmid[k]) for k in active}):
         S(f'const bool hh{_tag(tm)} = T > {lit(tm)};')
     for k in active:
-        S(f'sum_h += w{k} * (hh{_tag(tmid[k])} ? ({_hpoly(t["h_hi"][k])}) : ({_hpoly(t["h_lo"][k])}));')
+        S(f'sum_h += {wvar}{k} * (hh{_tag(tmid[k])} ? ({_hpoly(t["h_hi"][k])}) : ({_hpoly(t["h_lo"][k])}));')
 
 
 def _tag(tm):
@@ -401,6 +631,7 @@ def _emit_bk2_fp64(t, info, pstate=False):
     else:
         S('            const double Tref)', 0)
     S('{', 0)
+    S(POOL_MARK)
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
     S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
@@ -476,6 +707,7 @@ def _emit_thermo_fp64(t, info):
     S('               double* __restrict__ rho_out, double* __restrict__ cp,', 0)
     S('               double* __restrict__ rhoCp, const double Tref)', 0)
     S('{', 0)
+    S(POOL_MARK)
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
     S('const double T = Tref * state[id];')
@@ -514,29 +746,48 @@ HEADER = '''// GENERATED by kinetix_amd.gen.emit_hip for mechanism "{name}" -- d
 '''
 
 
-def _with_pool(name, fn, *args):
+POOL_MARK = '// @@pool-prologue@@'
+LDS_POOL_MAX_BYTES = 100 * 1024
+
+
+def _with_pool(name, fn, *args, lds=False, block=BLOCK):
     global _POOL
-    _POOL = ConstPool(name)
+    _POOL = ConstPool(name, lds=lds)
     try:
         body = fn(*args)
-        return _POOL.declaration() + body
+        if lds and len(_POOL.values) * 8 > LDS_POOL_MAX_BYTES:
+            # too large for LDS: re-emit reading the table from global memory
+            _POOL = ConstPool(name, lds=False)
+            body = fn(*args)
+        return _POOL.declaration() + body.replace('  ' + POOL_MARK, _POOL.prologue(block).rstrip('\n'))
     finally:
         _POOL = None
 
 
-def generate(tables, info):
+DEFAULT_OPTS = {'bk1_sched_window': 0, 'bk1_wform': True, 'mixed': None, 'lds_pool': True,
+                'mixed_lds_pool': False}
+
+
+def generate(tables, info, opts=None):
+    o = dict(DEFAULT_OPTS)
+    o.update(opts or {})
     parts = [HEADER.format(name=info['name'], n_species=info['n_species'],
                            n_active=info['n_active_species'], n_reactions=info['n_reactions'])]
-    parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info))
+    parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info, o['bk1_sched_window'], o['bk1_wform'], lds=o['lds_pool']))
     if 'cond' in tables:
-        parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False))
-        parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True))
+        parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False, lds=o['lds_pool']))
+        parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, lds=o['lds_pool']))
     parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
+    from . import emit_mixed
+    parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],
+                            lds=o['mixed_lds_pool']))
+    parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_fp64, tables, info, 0, True, 'mixed_valu',
+                            lds=o['lds_pool']))
     return '\n\n'.join(parts) + '\n'
 
 
-def write(out_dir, mech, tables):
+def write(out_dir, mech, tables, opts=None):
     from .tables import meta
-    src = generate(tables, meta(mech))
+    src = generate(tables, meta(mech), opts)
     with open(os.path.join(out_dir, 'kernels.hip'), 'w') as fh:
         fh.write(src)
