@@ -119,12 +119,7 @@ double kx_ref_mean_molecular_weight(kx_mech_t m);
  * assume, and writes the 16x16 product row-major to d_out (256 floats). */
 int kx_selftest_mfma_layout(float *d_out, void *stream);
 
-/*
- * Learned transport surrogate (no reference counterpart). A 55 -> 3x256 GELU -> 55 MLP
- * (inputs: normalised T, ln p, Y_k; outputs: ln mu, ln lambda, ln D_km) with BF16 weights
- * and FP32 accumulation on MFMA, trained by this library's own HIP kernels on states the
- * plain BK2 kernel labels. See kinetix_mi_surrogate.h.
- */
+/* Learned transport surrogate (no reference counterpart): see kinetix_mi_surrogate.h. */
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,74 @@
+/*
+ * kinetix_mi_surrogate.h -- C ABI of the learned transport surrogate (no counterpart in the
+ * reference; it sits beside kinetix::mixtureAvgTransportProps, benchmark/src/kinetix.cpp:815-841).
+ *
+ * Model: y = b(T, p, Y) + out_mean + out_std * MLP((x - in_mean) / in_std), with
+ * y = (ln mu, ln lambda, ln D_1m .. ln D_Nm), x = (normalised T, normalised ln p, Y_1..Y_N)
+ * padded to 64, b an FP32 physics baseline, MLP 64 -> 3x256 GELU -> 64. All buffers are HIP
+ * device pointers; all calls are asynchronous on `stream` (NULL = default stream).
+ */
+#ifndef KINETIX_MI_SURROGATE_H
+#define KINETIX_MI_SURROGATE_H
+
+#include <stdint.h>
+
+#include "kinetix_mi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mechanism-specific (generated) kernels ---- */
+
+/* Standardised MLP inputs xin[n][64], baselines base[n][64] and out-of-hull flags[n]
+ * (1 = an input lies outside [hull_lo, hull_hi] or is NaN). d_p: p [Pa] per state.
+ * norm = {in_mean[64], in_std[64], hull_lo[64], hull_hi[64]} (FP32). */
+int kx_sur_features(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
+                    const double *d_state, const float *d_norm, float *d_xin, float *d_base,
+                    int *d_flags, void *stream);
+
+/* Training targets tgt[n][64] = (ln label - base - out_mean) / out_std from the plain
+ * kernel's viscosity, conductivity and D_km (rows k*offset). ostat = {out_mean[64], out_std[64]}. */
+int kx_sur_targets(kx_mech_t m, int n_states, int offset, const double *d_visc,
+                   const double *d_cond, const double *d_Dkm, const float *d_base,
+                   const float *d_ostat, float *d_tgt, void *stream);
+
+/* Outputs from base + MLP residual for every state whose flag is 0 (flagged states are left
+ * for the plain-kernel fallback). d_Dkm may be NULL. */
+int kx_sur_outputs(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
+                   const double *d_state, const float *d_base, const float *d_resid,
+                   const float *d_ostat, const int *d_flags, double *d_viscosity,
+                   double *d_conductivity, double *d_rhoD, double *d_Dkm, void *stream);
+
+/* ---- mechanism-independent MLP kernels ---- */
+
+int kx_mlp_n_params(void);          /* FP32 parameter block length */
+int kx_mlp_n_weights_bf16(void);    /* BF16 weight block length */
+
+/* Training forward pass, batch B (multiple of 64): acts receives Z1 H1 Z2 H2 Z3 H3 (B x 256
+ * each) and Y (B x 64). */
+int kx_mlp_forward(const float *params, int B, const float *X, float *acts, void *stream);
+
+/* Gradient of mean((Y - T)^2) over the first 55 outputs into grads (params layout).
+ * work: 2 x B x 256 floats; loss_part: ceil(B * 64 / 256) floats of partial squared errors. */
+int kx_mlp_backward(const float *params, int B, const float *X, const float *acts,
+                    const float *T, float *grads, float *work, float *loss_part, void *stream);
+
+/* Adam step `step` (1-based) over the whole parameter block. */
+int kx_mlp_adam(float *params, const float *grads, float *m, float *v, float lr, float beta1,
+                float beta2, float eps, int step, void *stream);
+
+/* BF16 copy of the weight matrices (round to nearest even). */
+int kx_mlp_pack_bf16(const float *params, uint16_t *weights_bf16, void *stream);
+
+/* Inference Y[n][64] = MLP(X[n][64]) on BF16 MFMA with FP32 accumulation. */
+int kx_mlp_infer_bf16(const uint16_t *weights_bf16, const float *params, int n_states,
+                      const float *X, float *Y, void *stream);
+
+/* Device self-test of the v_mfma_f32_16x16x16_bf16 lane layout (256 floats). */
+int kx_selftest_mfma_bf16_layout(float *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KINETIX_MI_SURROGATE_H */

@@ -19,7 +19,7 @@ ARCH = 'gfx950'
 # -packed-fp32-ops: no packed FP32 (v_pk_*_f32) code generation.
 COMMON = ['-O3', '-std=c++17', f'--offload-arch={ARCH}', '-ffp-contract=fast',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
-HOST_SOURCES = ['kx_host.cpp', 'kx_selftest.hip']
+HOST_SOURCES = ['kx_host.cpp', 'kx_selftest.hip', 'kx_mlp.hip']
 
 
 def _run(cmd):

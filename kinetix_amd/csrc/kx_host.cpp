@@ -130,7 +130,8 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
                 std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
   }
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
-                        "kx_bk2_fp64_pstate", "kx_thermo_fp64"}) {
+                        "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
+                        "kx_sur_targets", "kx_sur_outputs"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
     else (void)hipGetLastError();
@@ -264,6 +265,53 @@ int kx_thermodynamic_props(kx_mech_t h, int n, int offsetT, int offset, double p
   const double* s = d_state;
   void* args[] = {&n, &oT, &o, &pressure_R, &s, &d_rho, &d_cp, &d_rhoCp, &Tref};
   return launch(m, "kx_thermo_fp64", n, stream, args);
+}
+
+// ---- learned transport surrogate: mechanism-specific kernels of the generated code object
+int kx_sur_features(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
+                    const double* d_state, const float* d_norm, float* d_xin, float* d_base,
+                    int* d_flags, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_p);
+  if (rc) return rc;
+  if (n > 0 && (!d_norm || !d_xin || !d_base || !d_flags))
+    return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  void* args[] = {&n, &oT, &o, &d_p, &d_state, &d_norm, &d_xin, &d_base, &d_flags, &Tref};
+  return launch(m, "kx_sur_features", n, stream, args);
+}
+
+int kx_sur_targets(kx_mech_t h, int n, int offset, const double* d_visc, const double* d_cond,
+                   const double* d_Dkm, const float* d_base, const float* d_ostat, float* d_tgt,
+                   void* stream)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (n < 0 || offset < n) return fail(KX_ERR_INVALID_ARG, "bad size/offset");
+  if (n > 0 && (!d_visc || !d_cond || !d_Dkm || !d_base || !d_ostat || !d_tgt))
+    return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long o = offset;
+  void* args[] = {&n, &o, &d_visc, &d_cond, &d_Dkm, &d_base, &d_ostat, &d_tgt};
+  return launch(m, "kx_sur_targets", n, stream, args);
+}
+
+int kx_sur_outputs(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
+                   const double* d_state, const float* d_base, const float* d_resid,
+                   const float* d_ostat, const int* d_flags, double* d_visc, double* d_cond,
+                   double* d_rhoD, double* d_Dkm, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_p);
+  if (rc) return rc;
+  if (n > 0 && (!d_base || !d_resid || !d_ostat || !d_flags || !d_visc || !d_cond || !d_rhoD))
+    return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  void* args[] = {&n, &oT, &o, &d_p, &d_state, &d_base, &d_resid, &d_ostat, &d_flags,
+                  &d_visc, &d_cond, &d_rhoD, &d_Dkm, &Tref};
+  return launch(m, "kx_sur_outputs", n, stream, args);
 }
 
 int kx_n_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_species : KX_ERR_NOT_INITIALIZED; }

@@ -783,6 +783,9 @@ def generate(tables, info, opts=None):
                             lds=o['mixed_lds_pool']))
     parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_fp64, tables, info, 0, True, 'mixed_valu',
                             lds=o['lds_pool']))
+    if 'cond' in tables:
+        from . import emit_surrogate
+        parts.append(_with_pool('kx_sur_k', emit_surrogate.emit, tables, info))
     return '\n\n'.join(parts) + '\n'
 
 

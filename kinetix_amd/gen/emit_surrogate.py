@@ -1,0 +1,213 @@
+"""Mechanism-specific kernels of the learned transport surrogate (gfx950 HIP).
+
+The surrogate predicts the mixture-averaged transport outputs y = (ln mu, ln lambda,
+ln D_1m .. ln D_Nm) as  y = b(T, p, Y) + mean + std * MLP(x),  where b is a cheap FP32
+physics baseline and the MLP (55 -> 3x256 GELU -> 55, BF16 MFMA inference; hand-written
+kernels in kinetix_amd/csrc/kx_mlp.hip) learns the standardised residual:
+
+  * mu baseline: the Wilke mixing rule (the formula the reference uses,
+    kinetix/core/mix_transport.py:297-357) with its three constant pair matrices C1^2,
+    C1 C2, C2^2 replaced by rank-r factorisations (their singular values fall by ~30x per
+    rank; r = 8 reproduces them to ~1e-9 relative), so the O(N^2) pair sum costs O(r N);
+  * lambda baseline: the reference's O(N) conductivity mixing rule (mix_transport.py:261-294);
+  * D_km baseline: the binary diffusion coefficient of species k in N2 (N2 in O2), at the
+    state's T and p, from the same pair fits.
+
+Kernels:
+  kx_sur_features  state -> standardised MLP input (n x 64, FP32), baseline (n x 64) and an
+                   out-of-hull flag per state (input outside the training box)
+  kx_sur_targets   plain-kernel labels -> standardised training targets (n x 64)
+  kx_sur_outputs   baseline + de-standardised MLP output -> mu, lambda, rho*D_km (and D_km),
+                   FP64, for states not flagged for the plain-kernel fallback
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import emit_hip as E
+
+IN = 64            # padded MLP input / output width
+RANK = 8
+BLOCK = 256
+
+
+def wilke_factors(t, rank=RANK):
+    C1, C2 = t['C1'], t['C2']
+    out = []
+    for M in (C1 * C1, C1 * C2, C2 * C2):
+        U, s, Vt = np.linalg.svd(M)
+        out.append((U[:, :rank] * s[:rank], Vt[:rank]))
+    return out
+
+
+def ref_pairs(species):
+    """Bath-gas partner of every species for the D baseline: N2 (O2 for N2 itself)."""
+    n2 = species.index('N2') if 'N2' in species else 0
+    alt = species.index('O2') if 'O2' in species else (1 if n2 == 0 else 0)
+    return [n2 if k != n2 else alt for k in range(len(species))]
+
+
+def baseline_numpy(t, species, T, p, Y):
+    """Reference implementation of the baselines (FP64) for tests and training checks."""
+    W = t['W']
+    lnT = np.log(T)
+    P = np.stack([lnT ** i for i in range(5)])
+    X = np.maximum(Y, 0) / W[:, None]
+    X /= X.sum(0)
+    eta = t['visc'] @ P
+    fac = wilke_factors(t)
+    # S_k = sum_j X_j (C1 + C2 eta_k/eta_j)^2 with each constant matrix in rank-r form
+    S = (fac[0][0] @ (fac[0][1] @ X)) + 2 * eta * (fac[1][0] @ (fac[1][1] @ (X / eta))) \
+        + eta ** 2 * (fac[2][0] @ (fac[2][1] @ (X / eta ** 2)))
+    mu = np.sqrt(T) * (X * eta ** 2 / S).sum(0)
+    lam_k = t['cond'] @ P
+    lam = np.sqrt(T) * 0.5 * ((X * lam_k).sum(0) + 1.0 / (X / lam_k).sum(0))
+    pr = ref_pairs(species)
+    d = np.array([t['diff'][k, pr[k]] for k in range(len(species))]) @ P
+    D = d * T ** 1.5 / p[None] if not t.get('rcp_diff', 0) else T ** 1.5 / (d * p[None])
+    return np.log(mu), np.log(lam), np.log(D)
+
+
+def emit(t, info):
+    N = int(t['n_species'])
+    if N + 2 > IN:
+        return ''                                   # surrogate width is fixed at 64
+    sp = info['species']
+    W, rcpW = t['W'], t['rcpW']
+    lit = E.lit
+    fac = wilke_factors(t)
+    pr = ref_pairs(sp)
+    rcp_diff = bool(int(t.get('rcp_diff', 0)))
+    S = E._Src()
+
+    def poly(c, v='lnT'):
+        return (f'{lit(c[0])} + {lit(c[1])} * {v} + {lit(c[2])} * {v}2 + {lit(c[3])} * {v}3 + '
+                f'{lit(c[4])} * {v}4')
+
+    def prologue(pstate_arg=True):
+        S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+        S('if (id >= n_states) return;')
+        S('const double T = Tref * state[id];')
+        S('const double p = p_state[id];')
+        S('const double lnT = log(T);')
+        S('const double lnT2 = lnT * lnT, lnT3 = lnT2 * lnT, lnT4 = lnT2 * lnT2;')
+
+    # ---------------- features
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_features(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('                const double* __restrict__ p_state, const double* __restrict__ state,', 0)
+    S('                const float* __restrict__ norm, float* __restrict__ xin,', 0)
+    S('                float* __restrict__ base, int* __restrict__ flags, const double Tref)', 0)
+    S('{', 0)
+    S(E.POOL_MARK)
+    prologue()
+    S('// norm: in_mean[64], in_std[64], hull_lo[64], hull_hi[64] (raw input units)')
+    S('float raw[%d];' % (N + 2))
+    S('raw[0] = (float)((T - 1400.0) * (1.0 / 1100.0));')
+    S(f'raw[1] = (float)((log(p) - {lit(math.log(101325.0))}) * (1.0 / 1.3));')
+    for k in range(N):
+        S(f'raw[{k + 2}] = (float)state[id + offsetT + {k} * offset];')
+    S('int out = 0;')
+    S(f'float* xr = xin + id * {IN};')
+    S(f'#pragma unroll')
+    S(f'for (int i = 0; i < {N + 2}; ++i) {{')
+    S(f'  out |= (raw[i] < norm[{2 * IN} + i]) | (raw[i] > norm[{3 * IN} + i]) | (raw[i] != raw[i]);')
+    S(f'  xr[i] = (raw[i] - norm[i]) / norm[{IN} + i];')
+    S('}')
+    S(f'for (int i = {N + 2}; i < {IN}; ++i) xr[i] = 0.0f;')
+    S('flags[id] = out;')
+    S('// baselines (FP32): mole fractions, Wilke viscosity via rank-%d factors, conductivity'
+      ' mixing rule, binary diffusion in the bath gas' % RANK)
+    S('const float lT = (float)lnT, lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
+    S('const float sqT = sqrtf((float)T);')
+    for k in range(N):
+        S(f'const float y{k} = fmaxf(0.0f, raw[{k + 2}]) * {float(rcpW[k])!r}f;')
+    S('const float rsum = 1.0f / (' + ' + '.join(f'y{k}' for k in range(N)) + ');')
+    for k in range(N):
+        S(f'const float x{k} = y{k} * rsum;')
+
+    def fpoly(c):
+        return (f'{float(c[0])!r}f + {float(c[1])!r}f * lT + {float(c[2])!r}f * lT2 + '
+                f'{float(c[3])!r}f * lT3 + {float(c[4])!r}f * lT4')
+    for k in range(N):
+        S(f'const float e{k} = {fpoly(t["visc"][k])};')
+        S(f'const float ie{k} = 1.0f / e{k};')
+    for m in range(RANK):
+        for a, (U, V) in enumerate(fac):
+            if a == 0:
+                terms = [f'{float(V[m, j])!r}f * x{j}' for j in range(N)]
+            elif a == 1:
+                terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j})' for j in range(N)]
+            else:
+                terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j} * ie{j})' for j in range(N)]
+            S(f'const float v{a}_{m} = ' + ' + '.join(terms) + ';')
+    S('float mu = 0.0f;')
+    for k in range(N):
+        s1 = ' + '.join(f'{float(fac[0][0][k, m])!r}f * v0_{m}' for m in range(RANK))
+        s2 = ' + '.join(f'{float(fac[1][0][k, m])!r}f * v1_{m}' for m in range(RANK))
+        s3 = ' + '.join(f'{float(fac[2][0][k, m])!r}f * v2_{m}' for m in range(RANK))
+        S(f'{{ const float Sk = ({s1}) + 2.0f * e{k} * ({s2}) + e{k} * e{k} * ({s3});')
+        S(f'  mu += x{k} * e{k} * e{k} / Sk; }}')
+    S('float l1 = 0.0f, l2 = 0.0f;')
+    for k in range(N):
+        S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} / l; }}')
+    S(f'float* br = base + id * {IN};')
+    S('br[0] = logf(sqT * mu);')
+    S('br[1] = logf(sqT * 0.5f * (l1 + 1.0f / l2));')
+    S('const float lT15p = 1.5f * lT - (float)log(p);')
+    for k in range(N):
+        d = t['diff'][k, pr[k]]
+        expr = f'-logf({fpoly(d)})' if rcp_diff else f'logf({fpoly(d)})'
+        S(f'br[{k + 2}] = lT15p + {expr};')
+    S(f'for (int i = {N + 2}; i < {IN}; ++i) br[i] = 0.0f;')
+    S('}', 0)
+
+    # ---------------- training targets from plain-kernel labels
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_targets(const int n_states, const long long offset,', 0)
+    S('               const double* __restrict__ visc, const double* __restrict__ cond,', 0)
+    S('               const double* __restrict__ Dkm, const float* __restrict__ base,', 0)
+    S('               const float* __restrict__ ostat, float* __restrict__ tgt)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S(f'const float* br = base + id * {IN};')
+    S(f'float* tr = tgt + id * {IN};')
+    S('// ostat: out_mean[64], out_std[64]')
+    S(f'tr[0] = (float)(log(visc[id]) - (double)br[0] - (double)ostat[0]) / ostat[{IN}];')
+    S(f'tr[1] = (float)(log(cond[id]) - (double)br[1] - (double)ostat[1]) / ostat[{IN + 1}];')
+    S(f'for (int k = 0; k < {N}; ++k)')
+    S(f'  tr[k + 2] = (float)(log(Dkm[k * offset + id]) - (double)br[k + 2] - (double)ostat[k + 2]) / ostat[{IN} + k + 2];')
+    S(f'for (int i = {N + 2}; i < {IN}; ++i) tr[i] = 0.0f;')
+    S('}', 0)
+
+    # ---------------- outputs
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_outputs(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('               const double* __restrict__ p_state, const double* __restrict__ state,', 0)
+    S('               const float* __restrict__ base, const float* __restrict__ resid,', 0)
+    S('               const float* __restrict__ ostat, const int* __restrict__ flags,', 0)
+    S('               double* __restrict__ viscosity, double* __restrict__ conductivity,', 0)
+    S('               double* __restrict__ rhoD, double* __restrict__ Dkm, const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states || flags[id]) return;')
+    S('const double T = Tref * state[id];')
+    S('const double p = p_state[id];')
+    S(f'const float* br = base + id * {IN};')
+    S(f'const float* rr = resid + id * {IN};')
+    S(f'viscosity[id] = exp((double)br[0] + (double)ostat[0] + (double)ostat[{IN}] * (double)rr[0]);')
+    S(f'conductivity[id] = exp((double)br[1] + (double)ostat[1] + (double)ostat[{IN + 1}] * (double)rr[1]);')
+    S('double rcpMbar = 0.0;')
+    for k in range(N):
+        S(f'rcpMbar += fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S(f'const double rho = p * {lit(1.0 / float(t["R"]))} / (T * rcpMbar);')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  const double D = exp((double)br[k + 2] + (double)ostat[k + 2] + (double)ostat[{IN} + k + 2] * (double)rr[k + 2]);')
+    S('  rhoD[k * offset + id] = rho * D;')
+    S('  if (Dkm) Dkm[k * offset + id] = D;')
+    S('}')
+    S('}', 0)
+    return S.text()

@@ -47,3 +47,44 @@ def sample(n, species, W, T_range=(1000.0, 2500.0), seed=BASE_SEED, p_range_atm=
         lo, hi = np.log(p_range_atm[0]), np.log(p_range_atm[1])
         p = ONE_ATM * np.exp(rng.uniform(lo, hi, size=n))
     return state, p
+
+
+def product_mass_fractions(species, W):
+    """Complete-combustion products of stoichiometric CH4/air: CO2 + 2 H2O + 7.52 N2."""
+    X = np.zeros(len(species))
+    for name, x in (('CO2', 1.0), ('H2O', 2.0), ('N2', 7.52)):
+        if name in species:
+            X[species.index(name)] = x
+    Y = X * W
+    return Y / Y.sum()
+
+
+def sample_blends(n, species, W, T_range=(300.0, 2500.0), p_range_atm=(0.5, 40.0), seed=BASE_SEED,
+                  radical_max=1e-2):
+    """Linear blends between CH4/air reactants and CO2/H2O/N2 products, plus the radicals
+    OH, H, O at U[0, radical_max] mass fraction each, renormalised."""
+    rng = np.random.default_rng(seed)
+    N = len(species)
+    T = rng.uniform(T_range[0], T_range[1], size=n)
+    s = rng.uniform(0.0, 1.0, size=n)
+    Ya, Yp = air_mass_fractions(species, W), product_mass_fractions(species, W)
+    Y = (1.0 - s)[None, :] * Ya[:, None] + s[None, :] * Yp[:, None]
+    for name in ('OH', 'H', 'O'):
+        if name in species:
+            Y[species.index(name)] += rng.uniform(0.0, radical_max, size=n)
+    Y /= Y.sum(axis=0, keepdims=True)
+    state = np.empty((N + 1, n))
+    state[0] = T
+    state[1:] = Y
+    lo, hi = np.log(p_range_atm[0]), np.log(p_range_atm[1])
+    p = ONE_ATM * np.exp(rng.uniform(lo, hi, size=n))
+    return state, p
+
+
+def sample_transport_set(n, species, W, seed=BASE_SEED):
+    """Surrogate training/inference distribution: half Dirichlet/air mixtures with
+    T ~ U[300, 2500] K and p ~ logU[0.5, 40] atm, half reactant/product blends."""
+    h = n // 2
+    s1, p1 = sample(h, species, W, T_range=(300.0, 2500.0), seed=seed, p_range_atm=(0.5, 40.0))
+    s2, p2 = sample_blends(n - h, species, W, seed=seed + 7919)
+    return np.concatenate([s1, s2], axis=1), np.concatenate([p1, p2])

@@ -1,0 +1,316 @@
+"""Learned transport surrogate: training and audited inference (host side).
+
+Model (per state): y = b(T, p, Y) + out_mean + out_std * MLP((x - in_mean) / in_std), with
+y = (ln mu, ln lambda, ln D_1m .. ln D_Nm) and b the FP32 physics baseline of
+kinetix_amd/gen/emit_surrogate.py. All arithmetic runs in this library's HIP kernels
+(include/kinetix_mi_surrogate.h): features, targets, FP32 training GEMMs/Adam, BF16-MFMA
+inference, outputs, and the plain FP64 transport kernel for labels, audits and fallback.
+torch provides device memory, streams, random permutations and index gathers only.
+
+Inference safeguards (the fallback route is the plain kernel, kx_mixture_avg_transport_pstate):
+  * hull check: any input outside the box spanned by the training inputs (small margin)
+    -> plain kernel;
+  * audit: every `audit_stride`-th state (1 % by default) is also computed by the plain
+    kernel; an audited state whose mu or lambda differs by more than `audit_tol` takes the
+    plain result; if more than `max_audit_fail` of the audited states fail, the whole call
+    falls back to the plain kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import states as st
+from .api import Kinetix, KinetixError, _check, lib
+
+IN = 64
+VALID = 55
+
+
+def _bind():
+    L = lib()
+    vp, ci, cf = C.c_void_p, C.c_int, C.c_float
+    L.kx_sur_features.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_sur_targets.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_sur_outputs.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_mlp_forward.argtypes = [vp, ci, vp, vp, vp]
+    L.kx_mlp_backward.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
+    L.kx_mlp_pack_bf16.argtypes = [vp, vp, vp]
+    L.kx_mlp_infer_bf16.argtypes = [vp, vp, ci, vp, vp, vp]
+    return L
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _s():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def init_params(seed=0):
+    """Glorot-uniform weights, zero biases; padded input columns / output rows are zero."""
+    rng = np.random.default_rng(seed)
+    n = lib().kx_mlp_n_params()
+    p = np.zeros(n, dtype=np.float32)
+    o = 0
+    for (rows, cols, vr, vc) in ((256, 64, 256, VALID), (256, 256, 256, 256), (256, 256, 256, 256),
+                                 (64, 256, VALID, 256)):
+        lim = math.sqrt(6.0 / (vr + vc))
+        W = np.zeros((rows, cols), dtype=np.float32)
+        W[:vr, :vc] = rng.uniform(-lim, lim, size=(vr, vc))
+        p[o:o + rows * cols] = W.ravel()
+        o += rows * cols + rows              # weights then bias
+    assert o == n
+    return p
+
+
+class Surrogate:
+    """A trained surrogate bound to one loaded mechanism."""
+
+    def __init__(self, kx: Kinetix, model=None):
+        import torch
+        self.kx = kx
+        self.L = _bind()
+        self.N = kx.nSpecies()
+        if self.N + 2 > IN:
+            raise KinetixError('surrogate supports at most 62 species')
+        self.dev = torch.device('cuda')
+        if model is not None:
+            self.load(model)
+
+    # ---------------------------------------------------------------- model state
+    def set_model(self, params, norm, ostat, meta=None):
+        import torch
+        self.params = torch.as_tensor(np.asarray(params, np.float32), device=self.dev).contiguous()
+        self.norm = torch.as_tensor(np.asarray(norm, np.float32), device=self.dev).contiguous()
+        self.ostat = torch.as_tensor(np.asarray(ostat, np.float32), device=self.dev).contiguous()
+        self.wbf16 = torch.empty(self.L.kx_mlp_n_weights_bf16(), dtype=torch.int16, device=self.dev)
+        _check(self.L.kx_mlp_pack_bf16(_p(self.params), _p(self.wbf16), _s()))
+        self.meta = meta or {}
+
+    def save(self, path):
+        np.savez_compressed(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy(),
+                            ostat=self.ostat.cpu().numpy(), meta=json.dumps(self.meta))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_model(z['params'], z['norm'], z['ostat'], json.loads(str(z['meta'])))
+
+    # ---------------------------------------------------------------- kernels
+    def features(self, n, offsetT, offset, d_p, d_state, norm=None):
+        import torch
+        xin = torch.empty((n, IN), dtype=torch.float32, device=self.dev)
+        base = torch.empty((n, IN), dtype=torch.float32, device=self.dev)
+        flags = torch.empty(n, dtype=torch.int32, device=self.dev)
+        _check(self.L.kx_sur_features(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state),
+                                      _p(self.norm if norm is None else norm), _p(xin), _p(base),
+                                      _p(flags), _s()))
+        return xin, base, flags
+
+    def plain(self, n, offsetT, offset, d_p, d_state, Dkm=True):
+        import torch
+        z = lambda m: torch.empty(m, dtype=torch.float64, device=self.dev)
+        visc, cond, rhoD = z(n), z(n), z(self.N * offset)
+        dkm = z(self.N * offset) if Dkm else None
+        self.kx.mixtureAvgTransportPropsPstate(n, offsetT, offset, d_p, d_state, visc, cond, rhoD, dkm)
+        return visc, cond, rhoD, dkm
+
+    # ---------------------------------------------------------------- training
+    def train(self, n_train=1 << 24, epochs=20, batch=8192, lr=2e-3, seed=st.BASE_SEED + 4,
+              log=print, holdout=65536, hull_margin=0.02):
+        import torch
+        N, dev = self.N, self.dev
+        sp, W = self.kx.speciesNames(), self.kx.molar_masses
+        t0 = time.time()
+        # training states (host-side seeded sampler, uploaded in chunks), labels from the
+        # plain FP64 kernel on the GPU
+        n = n_train + holdout
+        d_state = torch.empty((N + 1) * n, dtype=torch.float64, device=dev)
+        d_p = torch.empty(n, dtype=torch.float64, device=dev)
+        view = d_state.view(N + 1, n)
+        chunk = 1 << 21
+        for c0 in range(0, n, chunk):
+            m = min(chunk, n - c0)
+            s_, p_ = st.sample_transport_set(m, sp, W, seed=seed + c0)
+            view[:, c0:c0 + m].copy_(torch.from_numpy(s_))
+            d_p[c0:c0 + m].copy_(torch.from_numpy(p_))
+        visc, cond, _, dkm = self.plain(n, n, n, d_p, d_state)
+        torch.cuda.synchronize()
+        log(f'surrogate: {n} states + plain-kernel labels in {time.time() - t0:.1f} s')
+        # input statistics and hull from the raw features (identity normalisation)
+        ident = torch.zeros(4 * IN, dtype=torch.float32, device=dev)
+        ident[IN:2 * IN] = 1.0
+        ident[2 * IN:3 * IN] = -float('inf')
+        ident[3 * IN:] = float('inf')
+        raw, base, _ = self.features(n, n, n, d_p, d_state, norm=ident)
+        tr = raw[:n_train]
+        mean, std = tr.mean(0), tr.std(0)
+        lo, hi = tr.min(0).values, tr.max(0).values
+        span = (hi - lo)
+        norm = torch.cat([torch.where(std > 0, mean, torch.zeros_like(mean)),
+                          torch.where(std > 1e-12, std, torch.ones_like(std)),
+                          lo - hull_margin * span, hi + hull_margin * span]).contiguous()
+        del raw
+        xin, base, _ = self.features(n, n, n, d_p, d_state, norm=norm)
+        # output statistics of the residuals, then standardised targets
+        ost0 = torch.zeros(2 * IN, dtype=torch.float32, device=dev)
+        ost0[IN:] = 1.0
+        tgt = torch.empty((n, IN), dtype=torch.float32, device=dev)
+        _check(self.L.kx_sur_targets(self.kx.h, n, n, _p(visc), _p(cond), _p(dkm), _p(base),
+                                     _p(ost0), _p(tgt), _s()))
+        om, osd = tgt[:n_train].mean(0), tgt[:n_train].std(0)
+        osd = torch.clamp(osd, min=1e-6)
+        om[VALID:] = 0.0
+        osd[VALID:] = 1.0
+        ostat = torch.cat([om, osd]).contiguous()
+        _check(self.L.kx_sur_targets(self.kx.h, n, n, _p(visc), _p(cond), _p(dkm), _p(base),
+                                     _p(ostat), _p(tgt), _s()))
+        torch.cuda.synchronize()
+        log(f'surrogate: features/targets ready at {time.time() - t0:.1f} s')
+
+        params = torch.as_tensor(init_params(seed), device=dev).contiguous()
+        P = params.numel()
+        grads = torch.zeros(P, dtype=torch.float32, device=dev)
+        m1, m2 = torch.zeros_like(grads), torch.zeros_like(grads)
+        acts = torch.empty(6 * batch * 256 + batch * IN, dtype=torch.float32, device=dev)
+        work = torch.empty(2 * batch * 256, dtype=torch.float32, device=dev)
+        part = torch.empty((batch * IN + 255) // 256, dtype=torch.float32, device=dev)
+        steps_per_epoch = n_train // batch
+        total = epochs * steps_per_epoch
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        step = 0
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(n_train, generator=gen, device=dev)
+            ep_loss = torch.zeros((), dtype=torch.float64, device=dev)
+            for b in range(steps_per_epoch):
+                idx = perm[b * batch:(b + 1) * batch]
+                Xb = xin.index_select(0, idx)
+                Tb = tgt.index_select(0, idx)
+                step += 1
+                # one-cycle schedule: linear warm-up over 5 %, cosine decay to 1 % of lr
+                f = step / total
+                lr_t = lr * (f / 0.05 if f < 0.05 else 0.01 + 0.99 * 0.5 * (1 + math.cos(math.pi * (f - 0.05) / 0.95)))
+                _check(self.L.kx_mlp_forward(_p(params), batch, _p(Xb), _p(acts), _s()))
+                _check(self.L.kx_mlp_backward(_p(params), batch, _p(Xb), _p(acts), _p(Tb), _p(grads),
+                                              _p(work), _p(part), _s()))
+                _check(self.L.kx_mlp_adam(_p(params), _p(grads), _p(m1), _p(m2), lr_t, 0.9, 0.999,
+                                          1e-8, step, _s()))
+                ep_loss += part.sum()
+            l_ = float(ep_loss) / (steps_per_epoch * batch * VALID)
+            hist.append(l_)
+            log(f'surrogate: epoch {ep + 1}/{epochs} mse {l_:.3e} ({time.time() - t0:.1f} s)')
+        meta = {'n_train': n_train, 'epochs': epochs, 'batch': batch, 'lr': lr, 'seed': seed,
+                'loss_history': hist, 'train_seconds': time.time() - t0,
+                'mechanism': os.path.basename(self.kx.dir)}
+        self.set_model(params.cpu().numpy(), norm.cpu().numpy(), ostat.cpu().numpy(), meta)
+        return meta
+
+    # ---------------------------------------------------------------- inference
+    def infer(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None,
+              audit_stride=100, audit_tol=1e-3, max_audit_fail=0.01, audit_offset=0):
+        """Surrogate transport for n states (FP64 outputs in the plain kernel's layout).
+        Returns a dict with the fallback statistics."""
+        import torch
+        N, dev = self.N, self.dev
+        if n == 0:
+            return {'n': 0, 'fallback': 0, 'fallback_fraction': 0.0}
+        xin, base, flags = self.features(n, offsetT, offset, d_p, d_state)
+        resid = torch.empty((n, IN), dtype=torch.float32, device=dev)
+        _check(self.L.kx_mlp_infer_bf16(_p(self.wbf16), _p(self.params), n, _p(xin), _p(resid), _s()))
+        _check(self.L.kx_sur_outputs(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state), _p(base),
+                                     _p(resid), _p(self.ostat), _p(flags), _p(d_visc), _p(d_cond),
+                                     _p(d_rhoD), _p(d_Dkm), _s()))
+        out_of_hull = int(flags.sum())
+        # audit: every audit_stride-th state through the plain kernel
+        aidx = torch.arange(audit_offset % max(audit_stride, 1), n, max(audit_stride, 1), device=dev)
+        n_fail = 0
+        if audit_stride > 0 and aidx.numel():
+            g_state, g_p = self._gather(aidx, offsetT, offset, d_state, d_p)
+            m = aidx.numel()
+            v, c, r, d = self.plain(m, m, m, g_p, g_state, Dkm=d_Dkm is not None)
+            ev = (d_visc.index_select(0, aidx) / v - 1).abs()
+            ec = (d_cond.index_select(0, aidx) / c - 1).abs()
+            bad = ((ev > audit_tol) | (ec > audit_tol) | ~torch.isfinite(ev) | ~torch.isfinite(ec))
+            bad = bad & (flags.index_select(0, aidx) == 0)
+            n_fail = int(bad.sum())
+            # audited states always take the plain result
+            self._scatter(aidx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm)
+        audited = int(aidx.numel()) if audit_stride > 0 else 0
+        whole = audited > 0 and n_fail > max_audit_fail * audited
+        if whole:
+            v, c, r, d = self.plain(n, offsetT, offset, d_p, d_state, Dkm=d_Dkm is not None)
+            d_visc.copy_(v); d_cond.copy_(c); d_rhoD[:r.numel()].copy_(r)
+            if d_Dkm is not None:
+                d_Dkm[:d.numel()].copy_(d)
+            n_fb = n
+        else:
+            fidx = torch.nonzero(flags, as_tuple=False).flatten()
+            if fidx.numel():
+                g_state, g_p = self._gather(fidx, offsetT, offset, d_state, d_p)
+                m = fidx.numel()
+                v, c, r, d = self.plain(m, m, m, g_p, g_state, Dkm=d_Dkm is not None)
+                self._scatter(fidx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm)
+            n_fb = out_of_hull + n_fail
+        return {'n': n, 'out_of_hull': out_of_hull, 'audited': audited, 'audit_failed': n_fail,
+                'whole_call_fallback': bool(whole), 'fallback': n_fb,
+                'fallback_fraction': n_fb / n}
+
+    def _gather(self, idx, offsetT, offset, d_state, d_p):
+        import torch
+        N = self.N
+        m = idx.numel()
+        g = torch.empty((N + 1, m), dtype=torch.float64, device=self.dev)
+        g[0] = d_state.index_select(0, idx)
+        rows = offsetT + torch.arange(N, device=self.dev)[:, None] * offset + idx[None, :]
+        g[1:] = d_state[rows]
+        return g.reshape(-1), d_p.index_select(0, idx)
+
+    def _scatter(self, idx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm):
+        import torch
+        N = self.N
+        d_visc.index_copy_(0, idx, v)
+        d_cond.index_copy_(0, idx, c)
+        rows = (torch.arange(N, device=self.dev)[:, None] * offset + idx[None, :]).reshape(-1)
+        d_rhoD.index_copy_(0, rows, r.view(N, m).reshape(-1))
+        if d_Dkm is not None:
+            d_Dkm.index_copy_(0, rows, d.view(N, m).reshape(-1))
+
+
+def model_path(mech='gri30'):
+    from .api import MECH_ROOT
+    return os.path.join(MECH_ROOT, mech, 'surrogate.npz')
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description='train the transport surrogate on the GPU')
+    ap.add_argument('--mech', default='gri30')
+    ap.add_argument('--n-train', type=int, default=1 << 24)
+    ap.add_argument('--epochs', type=int, default=20)
+    ap.add_argument('--batch', type=int, default=8192)
+    ap.add_argument('--lr', type=float, default=2e-3)
+    ap.add_argument('--out', default=None)
+    a = ap.parse_args(argv)
+    kx = Kinetix(a.mech)
+    kx.build(st.ONE_ATM, 1.0, st.air_mass_fractions(kx.speciesNames(), kx.molar_masses))
+    sur = Surrogate(kx)
+    meta = sur.train(n_train=a.n_train, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                     log=lambda s: print(s, flush=True))
+    out = a.out or model_path(a.mech)
+    sur.save(out)
+    print(json.dumps({'saved': out, 'train_seconds': meta['train_seconds'],
+                      'final_mse': meta['loss_history'][-1]}), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -20,3 +20,14 @@ def test_mfma_f32_16x16x4_layout():
     A = np.array([[i + 16 * k + 1 for k in range(4)] for i in range(16)], dtype=np.float64)
     B = np.array([[(k + 1) * 100 + j for j in range(16)] for k in range(4)], dtype=np.float64)
     assert np.array_equal(out.cpu().numpy().reshape(16, 16).astype(np.float64), A @ B)
+
+
+def test_mfma_bf16_16x16x16_layout():
+    L = lib()
+    L.kx_selftest_mfma_bf16_layout.argtypes = [C.c_void_p, C.c_void_p]
+    out = torch.full((256,), np.nan, dtype=torch.float32, device='cuda')
+    assert L.kx_selftest_mfma_bf16_layout(C.c_void_p(out.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    A = np.array([[(i + 16 * k + 1) / 16.0 for k in range(16)] for i in range(16)])
+    B = np.array([[(k + 1) * 3 + j for j in range(16)] for k in range(16)], dtype=np.float64)
+    assert np.array_equal(out.cpu().numpy().reshape(16, 16).astype(np.float64), A @ B)

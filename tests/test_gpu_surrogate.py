@@ -1,0 +1,186 @@
+"""The surrogate's hand-written training kernels against the FP64 CPU oracle (forward,
+gradients, Adam), BF16-MFMA inference against the FP32 forward, and the surrogate's
+fallback routes (hull check, audit, corrupted weights) against the plain FP64 kernel."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from kinetix_amd import states
+from kinetix_amd.api import Kinetix
+from kinetix_amd.surrogate import IN, Surrogate, _bind, _p, init_params
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+dev = torch.device('cuda')
+
+
+def t32(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
+
+
+@pytest.fixture(scope='module')
+def L():
+    return _bind()
+
+
+@pytest.fixture(scope='module')
+def case():
+    rng = np.random.default_rng(3)
+    p = init_params(1)
+    p = p + rng.normal(0, 0.01, size=p.shape).astype(np.float32) * (p != 0)
+    X = rng.normal(size=(128, IN)).astype(np.float32)
+    X[:, 55:] = 0.0
+    T = rng.normal(size=(128, IN)).astype(np.float32)
+    return p, X, T
+
+
+def test_forward_and_gradients_vs_oracle(L, case):
+    p, X, T = case
+    B = X.shape[0]
+    dp, dX, dT = t32(p), t32(X), t32(T)
+    acts = torch.empty(6 * B * 256 + B * IN, dtype=torch.float32, device=dev)
+    grads = torch.empty_like(dp)
+    work = torch.empty(2 * B * 256, dtype=torch.float32, device=dev)
+    part = torch.empty((B * IN + 255) // 256, dtype=torch.float32, device=dev)
+    assert L.kx_mlp_forward(_p(dp), B, _p(dX), _p(acts), None) == 0
+    assert L.kx_mlp_backward(_p(dp), B, _p(dX), _p(acts), _p(dT), _p(grads), _p(work), _p(part), None) == 0
+    torch.cuda.synchronize()
+    loss, Y, g = mo.loss_and_grad(p, X, T)
+    Yg = acts[6 * B * 256:].view(B, IN).cpu().numpy()
+    assert np.max(np.abs(Yg - Y)) <= 1e-5 * np.max(np.abs(Y))
+    gg = grads.cpu().numpy()
+    assert np.max(np.abs(gg - g)) <= 1e-4 * np.max(np.abs(g))
+    assert abs(float(part.sum()) / (B * 55) - loss) <= 1e-5 * loss
+
+
+def test_adam_vs_oracle(L, case):
+    p, _, _ = case
+    rng = np.random.default_rng(4)
+    g = rng.normal(size=p.shape).astype(np.float32)
+    m = rng.normal(size=p.shape).astype(np.float32) * 0.1
+    v = np.abs(rng.normal(size=p.shape)).astype(np.float32) * 0.01
+    dp, dg, dm, dv = t32(p), t32(g), t32(m), t32(v)
+    assert L.kx_mlp_adam(_p(dp), _p(dg), _p(dm), _p(dv), 1e-3, 0.9, 0.999, 1e-8, 7, None) == 0
+    torch.cuda.synchronize()
+    # the C ABI takes FP32 hyper-parameters: the oracle gets the same rounded values
+    f32 = lambda x: float(np.float32(x))
+    pr, mr, vr = mo.adam_step(*(a.astype(np.float64) for a in (p, g, m, v)), f32(1e-3), f32(0.9),
+                              f32(0.999), f32(1e-8), 7)
+    assert np.max(np.abs(dp.cpu().numpy() - pr)) <= 1e-6
+    assert np.max(np.abs(dm.cpu().numpy() - mr)) <= 1e-6
+    assert np.max(np.abs(dv.cpu().numpy() - vr)) <= 1e-7
+
+
+@pytest.mark.parametrize('n', [128, 1000, 1])
+def test_bf16_inference_vs_fp32(L, case, n):
+    p, _, _ = case
+    rng = np.random.default_rng(n)
+    X = rng.normal(size=(n, IN)).astype(np.float32)
+    X[:, 55:] = 0.0
+    dp = t32(p)
+    wb = torch.empty(L.kx_mlp_n_weights_bf16(), dtype=torch.int16, device=dev)
+    assert L.kx_mlp_pack_bf16(_p(dp), _p(wb), None) == 0
+    Y = torch.full((n, IN), np.nan, dtype=torch.float32, device=dev)
+    assert L.kx_mlp_infer_bf16(_p(wb), _p(dp), n, _p(t32(X)), _p(Y), None) == 0
+    torch.cuda.synchronize()
+    ref, _ = mo.forward(p, X)
+    err = np.abs(Y.cpu().numpy() - ref)
+    # BF16 weights and activations (8-bit mantissa), FP32 accumulation
+    assert np.all(np.isfinite(Y.cpu().numpy()))
+    assert err.max() <= 0.05 * np.abs(ref).max()
+    assert np.sqrt((err ** 2).mean()) <= 0.01 * np.sqrt((ref ** 2).mean())
+
+
+@pytest.fixture(scope='module')
+def trained(gri):
+    """A small surrogate trained on the GPU by the library's own kernels (smoke-sized)."""
+    info, t = gri
+    kx = Kinetix('gri30')
+    kx.build(states.ONE_ATM, 1.0, states.air_mass_fractions(info['species'], t['W']))
+    sur = Surrogate(kx)
+    meta = sur.train(n_train=1 << 17, epochs=4, batch=4096, holdout=8192, log=lambda s: None)
+    yield kx, sur, meta
+    kx.close()
+
+
+def _run(sur, kx, s, p, **kw):
+    N = kx.nSpecies()
+    n = s.shape[1]
+    ds = torch.from_numpy(np.ascontiguousarray(s)).to(dev).reshape(-1)
+    dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+    z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
+    v, c, r, d = z(n), z(n), z(N * n), z(N * n)
+    info = sur.infer(n, n, n, dp, ds, v, c, r, d, **kw)
+    pv, pc, pr, pd = sur.plain(n, n, n, dp, ds)
+    torch.cuda.synchronize()
+    return info, (v, c, r, d), (pv, pc, pr, pd)
+
+
+def test_training_reduces_loss(trained):
+    _, _, meta = trained
+    h = meta['loss_history']
+    assert h[-1] < 0.5 * h[0]
+
+
+def test_surrogate_accuracy_and_gain(trained, gri):
+    """mu and lambda within 1e-3 on every non-fallback state; the network must beat its own
+    physics baseline for D_km (the gain of the learned part)."""
+    info, t = gri
+    kx, sur, _ = trained
+    s, p = states.sample_transport_set(20000, info['species'], t['W'], seed=99)
+    res, (v, c, r, d), (pv, pc, pr, pd) = _run(sur, kx, s, p)
+    ok = torch.ones(s.shape[1], dtype=torch.bool, device=dev)
+    ev = (v / pv - 1).abs()
+    ec = (c / pc - 1).abs()
+    assert float(ev.max()) <= 1e-3 and float(ec.max()) <= 1e-3
+    N = kx.nSpecies()
+    n = s.shape[1]
+    ed_net = (d.view(N, n) / pd.view(N, n) - 1).abs()
+    # baseline-only prediction: residual forced to zero
+    xin, base, flags = sur.features(n, n, n, torch.from_numpy(p).to(dev),
+                                    torch.from_numpy(s).to(dev).reshape(-1))
+    lnD_base = base[:, 2:2 + N].double().T + sur.ostat[2:2 + N].double()[:, None]
+    ed_base = (torch.exp(lnD_base) / pd.view(N, n) - 1).abs()
+    assert float(ed_net.median()) < 0.5 * float(ed_base.median())
+    assert res['fallback_fraction'] < 0.05
+
+
+def test_out_of_hull_states_take_the_plain_result(trained, gri):
+    info, t = gri
+    kx, sur, _ = trained
+    s, p = states.sample_transport_set(1000, info['species'], t['W'], seed=5)
+    s[0, :10] = 3500.0                          # beyond the training temperatures
+    res, got, ref = _run(sur, kx, s, p, audit_stride=0)
+    assert res['out_of_hull'] >= 10
+    for a, b in zip(got, ref):
+        a_, b_ = a.view(-1, 1000)[:, :10], b.view(-1, 1000)[:, :10]
+        assert torch.equal(a_, b_)
+
+
+def test_audit_failure_falls_back_whole_call(trained, gri):
+    info, t = gri
+    kx, sur, _ = trained
+    s, p = states.sample_transport_set(3000, info['species'], t['W'], seed=6)
+    res, got, ref = _run(sur, kx, s, p, audit_tol=0.0)       # every audited state "fails"
+    assert res['whole_call_fallback'] and res['fallback_fraction'] == 1.0
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+
+
+def test_corrupted_weights_are_caught(trained, gri):
+    info, t = gri
+    kx, sur, _ = trained
+    good = sur.params.clone()
+    try:
+        bad = good.cpu().numpy().copy()
+        bad[:1000] = np.nan
+        sur.set_model(bad, sur.norm.cpu().numpy(), sur.ostat.cpu().numpy(), sur.meta)
+        s, p = states.sample_transport_set(3000, info['species'], t['W'], seed=7)
+        res, got, ref = _run(sur, kx, s, p)
+        assert res['whole_call_fallback']
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b)
+    finally:
+        sur.set_model(good.cpu().numpy(), sur.norm.cpu().numpy(), sur.ostat.cpu().numpy(), sur.meta)
