@@ -26,6 +26,8 @@ METRIC = 'Mstates/s for rates & transport (GRI-3.0, 4M states) at stated err vs 
 PEAK_FP64 = 78.6      # TFLOP/s, MI355X FP64 (vector = matrix)
 PEAK_FP32 = 157.3     # TFLOP/s
 PEAK_HBM = 8000.0     # GB/s
+PEAK_BF16 = 2500.0    # TFLOP/s, dense BF16 MFMA
+MLP_FLOPS = 2 * (55 * 256 + 256 * 256 * 2 + 256 * 55)   # useful MACs x 2 per state
 P_ATM = 101325.0
 
 
@@ -39,7 +41,11 @@ def parse():
     ap.add_argument('--bk1', choices=('fp64', 'mixed', 'mixed_mfma'), default='mixed',
                     help='headline BK1 kernel: fp64, mixed (FP32 rates, FP64 K_c/net/sum, '
                          'VALU accumulation) or mixed_mfma (q.nu on FP32 MFMA)')
-    ap.add_argument('--bk2', choices=('fp64', 'surrogate'), default='fp64')
+    ap.add_argument('--bk2', choices=('fp64', 'surrogate'), default='surrogate',
+                    help='headline BK2: fp64 (plain kernel) or surrogate (BF16-MFMA MLP with hull '
+                         'check, 1%% audit and fallback to the plain kernel)')
+    ap.add_argument('--surrogate-model', default=None,
+                    help='trained model (.npz); default kinetix_amd/mechs/<mech>/surrogate.npz')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=10.0,
                     help='target wall time of the bounded CPU-baseline sample')
@@ -125,13 +131,31 @@ def main():
     BK1 = {'fp64': ('kx_bk1_fp64', False), 'mixed': ('kx_bk1_mixed_valu', 'valu'),
            'mixed_mfma': ('kx_bk1_mixed', 'mfma')}
     bk1_name, bk1_mode = BK1[a.bk1]
-    bk2_name = 'kx_bk2_fp64'
+    bk2_name = 'kx_bk2_fp64' if a.bk2 == 'fp64' else 'kx_mlp_infer_bf16'
+    d_p = torch.full((n,), P_ATM, dtype=torch.float64, device=dev)
+    sur = None
+    if a.bk2 == 'surrogate':
+        from kinetix_amd.surrogate import Surrogate, model_path
+        path = a.surrogate_model or model_path(a.mech)
+        if not os.path.exists(path):
+            print(f'bench.py: no trained surrogate at {path}; train one with '
+                  f'python -m kinetix_amd.surrogate or run --bk2 fp64', file=sys.stderr)
+            return 2
+        sur = Surrogate(kx, path)
+    sur_stats = []
 
     def bk1(mode=None):
         kx.productionRates(n, n, n, 1.0, d_state, d_rates, mixed=bk1_mode if mode is None else mode)
 
-    def bk2():
+    def bk2_plain():
         kx.mixtureAvgTransportProps(n, n, n, 1.0, d_state, d_visc, d_cond, d_rhoD)
+
+    def bk2():
+        if sur is None:
+            bk2_plain()
+        else:
+            sur_stats.append(sur.infer(n, n, n, d_p, d_state, d_visc, d_cond, d_rhoD,
+                                       audit_offset=len(sur_stats)))
 
     if a.profile_only:
         # calibration: a plain copy moves exactly nbytes in and nbytes out
@@ -140,8 +164,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         bk1(); torch.cuda.synchronize()
-        bk2(); torch.cuda.synchronize()
-        print(f'profile-only: copy {src.numel() * 8} B, {bk1_name} and {bk2_name} over {n} states')
+        bk2_plain(); torch.cuda.synchronize()
+        if sur is not None:
+            bk2(); torch.cuda.synchronize()
+        print(f'profile-only: copy {src.numel() * 8} B, {bk1_name}, kx_bk2_fp64'
+              f'{" and the surrogate" if sur else ""} over {n} states')
         return 0
 
     for _ in range(a.warmup):
@@ -183,6 +210,42 @@ def main():
         v['speedup_vs_fp64'] = round(variants['fp64']['ms'] / v['ms'], 3)
     bk1(); torch.cuda.synchronize()               # leave the headline variant's output
 
+    # BK2 forms on the same inputs; the plain FP64 kernel is the judge of the surrogate over
+    # every state
+    def timed(fn, reps=5):
+        fn(); torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record(); torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    bk2_variants = {'fp64': {'kernel': 'kx_bk2_fp64', 'ms': round(timed(bk2_plain), 4)}}
+    p_visc, p_cond, p_rhoD = d_visc.clone(), d_cond.clone(), d_rhoD.clone()
+    sur_err = None
+    if sur is not None:
+        ms_s = timed(bk2)
+        st_ = sur_stats[-1]
+        ev_ = (d_visc / p_visc - 1).abs()
+        ec_ = (d_cond / p_cond - 1).abs()
+        er_ = (d_rhoD / p_rhoD - 1).abs()
+        sur_err = {'viscosity_max_rel_vs_plain': float(ev_.max()),
+                   'conductivity_max_rel_vs_plain': float(ec_.max()),
+                   'rhoD_max_rel_vs_plain': float(er_.max()),
+                   'rhoD_median_rel_vs_plain': float(er_.median()),
+                   'rhoD_p99_rel_vs_plain': float(torch.quantile(er_[::97].float(), 0.99)),
+                   'fallback_fraction': st_['fallback_fraction'],
+                   'out_of_hull': st_['out_of_hull'], 'audited': st_['audited'],
+                   'audit_failed': st_['audit_failed'],
+                   'whole_call_fallback': st_['whole_call_fallback'],
+                   'states_compared': n}
+        bk2_variants['surrogate'] = dict({'kernel': 'kx_mlp_infer_bf16 (+features/outputs, 1% audit)',
+                                          'ms': round(ms_s, 4)}, **sur_err)
+    for v in bk2_variants.values():
+        v['Mstates_s'] = round(n / v['ms'] / 1e3, 2)
+        v['speedup_vs_fp64'] = round(bk2_variants['fp64']['ms'] / v['ms'], 3)
+
     # errors of the headline outputs against the FP64 oracle on a sample of states
     rates = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
     c0, v0, r0, _ = orc.transport(st[:, idx], 1.0, threads=cores)
@@ -203,6 +266,15 @@ def main():
            'bk2_rhoD_max_rel': rel(rhoD, r0), 'sample_states': int(len(idx))}
 
     def roof(name, ms):
+        if name == 'kx_mlp_infer_bf16':
+            fl = MLP_FLOPS * n / (ms * 1e-3) / 1e12
+            return {'bound': 'mfma', 'pipe': 'BF16 MFMA (dense peak 2.5 PFLOP/s)',
+                    'achieved': round(fl, 3), 'peak': PEAK_BF16, 'unit': 'TFLOP/s',
+                    'frac': round(fl / PEAK_BF16, 4), 'traffic': None,
+                    'flops_per_state': MLP_FLOPS, 'avg_launch_ms': round(ms, 4),
+                    'states_per_launch': n,
+                    'note': 'avg_launch_ms is the whole surrogate call (features, MLP, outputs, '
+                            'audit); the MLP kernel alone is timed by rocprofv3'}
         c = counts[name if name in counts else 'kx_bk1_fp64']
         fl = c['flops'] * n / (ms * 1e-3) / 1e12
         gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
@@ -233,8 +305,10 @@ def main():
         'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
         'ms_per_step': round(ms_step, 4),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f64' if a.bk1 == 'fp64' else 'f32+f64',
+        'dtype': ('f64' if a.bk1 == 'fp64' else 'f32+f64') + ('+bf16' if sur is not None else ''),
+        'value_plain_fp64_path': round(n / (variants['fp64']['ms'] + bk2_variants['fp64']['ms']) / 1e3, 3),
         'bk1_variants': variants,
+        'bk2_variants': bk2_variants,
         'data': 'synthetic',
         'config': {'workload': f'{a.mech} BK1 production rates + BK2 mixture-averaged transport, '
                                f'{n} states, T~U[1000,2500] K, 1 atm, Y=0.5 Dirichlet(0.5)+0.5 CH4/air',
@@ -248,6 +322,8 @@ def main():
         'err_vs_fp64_oracle': err,
         'state_gen_s': round(gen_s, 2),
     }
+    if sur is not None:
+        out['surrogate'] = sur_err
     if not a.no_cpu:
         out['cpu_baseline'] = cpu_baseline(st, P_ATM, cores, a.cpu_seconds)
     print(json.dumps(out))

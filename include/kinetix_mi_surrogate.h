@@ -40,6 +40,17 @@ int kx_sur_outputs(kx_mech_t m, int n_states, int offsetT, int offset, const dou
                    const float *d_ostat, const int *d_flags, double *d_viscosity,
                    double *d_conductivity, double *d_rhoD, double *d_Dkm, void *stream);
 
+/* The whole inference path in one kernel: inputs, baselines and hull flags (flags[n], as in
+ * kx_sur_features), the MLP on BF16 MFMA (weights from kx_mlp_pack_bf16, biases from the
+ * FP32 parameter block; both 16-byte aligned) and the outputs of kx_sur_outputs, for every
+ * state whose flag is 0. Flagged states are left untouched for the plain-kernel fallback.
+ * d_Dkm may be NULL. This is what the audited inference (kinetix_amd/surrogate.py) calls. */
+int kx_sur_infer(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
+                 const double *d_state, const float *d_norm, const float *d_ostat,
+                 const uint16_t *d_weights_bf16, const float *d_params, int *d_flags,
+                 double *d_viscosity, double *d_conductivity, double *d_rhoD, double *d_Dkm,
+                 void *stream);
+
 /* ---- mechanism-independent MLP kernels ---- */
 
 int kx_mlp_n_params(void);          /* FP32 parameter block length */
@@ -58,15 +69,18 @@ int kx_mlp_backward(const float *params, int B, const float *X, const float *act
 int kx_mlp_adam(float *params, const float *grads, float *m, float *v, float lr, float beta1,
                 float beta2, float eps, int step, void *stream);
 
-/* BF16 copy of the weight matrices (round to nearest even). */
+/* BF16 copy of the weight matrices (round to nearest even) in the MFMA fragment order of
+ * the inference kernels (kx_mlp_n_weights_bf16() elements). */
 int kx_mlp_pack_bf16(const float *params, uint16_t *weights_bf16, void *stream);
 
 /* Inference Y[n][64] = MLP(X[n][64]) on BF16 MFMA with FP32 accumulation. */
 int kx_mlp_infer_bf16(const uint16_t *weights_bf16, const float *params, int n_states,
                       const float *X, float *Y, void *stream);
 
-/* Device self-test of the v_mfma_f32_16x16x16_bf16 lane layout (256 floats). */
+/* Device self-tests of the v_mfma_f32_16x16x16_bf16 and v_mfma_f32_16x16x32_bf16 lane
+ * layouts (256 floats each: D[i][j] at i * 16 + j). */
 int kx_selftest_mfma_bf16_layout(float *d_out, void *stream);
+int kx_selftest_mfma_bf16_k32_layout(float *d_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise KinetixError(f'{LIB_PATH} not built; run `python -m kinetix_amd.build`')
+        # torch bundles its own HIP runtime under the same soname (libamdhip64.so.7): load it
+        # first so that the library binds to that one runtime and torch's streams and device
+        # pointers are valid in both (loaded the other way round, torch finds no GPU).
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         L.kx_last_error.restype = C.c_char_p
         L.kx_init.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]

@@ -33,7 +33,8 @@ def _run(cmd):
 def build_mechanism(mech_dir, extra=()):
     src = os.path.join(mech_dir, 'kernels.hip')
     out = os.path.join(mech_dir, 'kernels.hsaco')
-    _run([HIPCC, '--genco', *COMMON, *extra, src, '-o', out])
+    inc = '-I' + os.path.join(PKG, 'csrc')          # kx_mlp_wave.h (surrogate inference)
+    _run([HIPCC, '--genco', *COMMON, inc, *extra, src, '-o', out])
     return out
 
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -131,7 +132,7 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
   }
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
-                        "kx_sur_targets", "kx_sur_outputs"}) {
+                        "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
     else (void)hipGetLastError();
@@ -312,6 +313,26 @@ int kx_sur_outputs(kx_mech_t h, int n, int offsetT, int offset, const double* d_
   void* args[] = {&n, &oT, &o, &d_p, &d_state, &d_base, &d_resid, &d_ostat, &d_flags,
                   &d_visc, &d_cond, &d_rhoD, &d_Dkm, &Tref};
   return launch(m, "kx_sur_outputs", n, stream, args);
+}
+
+int kx_sur_infer(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
+                 const double* d_state, const float* d_norm, const float* d_ostat,
+                 const uint16_t* d_wfrag, const float* d_params, int* d_flags, double* d_visc,
+                 double* d_cond, double* d_rhoD, double* d_Dkm, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_p);
+  if (rc) return rc;
+  if (n > 0 && (!d_norm || !d_ostat || !d_wfrag || !d_params || !d_flags || !d_visc || !d_cond ||
+                !d_rhoD))
+    return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  if ((reinterpret_cast<uintptr_t>(d_wfrag) | reinterpret_cast<uintptr_t>(d_params)) & 15)
+    return fail(KX_ERR_INVALID_ARG, "weights and parameters must be 16-byte aligned");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  void* args[] = {&n, &oT, &o, &d_p, &d_state, &d_norm, &d_ostat, &d_wfrag, &d_params, &d_flags,
+                  &d_visc, &d_cond, &d_rhoD, &d_Dkm, &Tref};
+  return launch(m, "kx_sur_infer", n, stream, args);
 }
 
 int kx_n_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_species : KX_ERR_NOT_INITIALIZED; }

@@ -3,27 +3,34 @@
 //
 // Training (FP32 master weights): tiled FP32 GEMM with either operand transposed, bias +
 // GELU forward, GELU backward, masked-MSE gradient, bias-gradient reduction, Adam.
-// Inference: one fused kernel per 128-state tile on BF16 MFMA (v_mfma_f32_16x16x16_bf16,
-// FP32 accumulation); activations ping-pong between two LDS buffers, weights stream
-// through LDS in K-chunks of 32.
+// Inference: BF16 weights packed into MFMA fragment order (kx_mlp_wave.h) and the
+// wave-level forward pass of kx_mlp_wave.h, 64 states per wavefront.
 //
 // Parameter block (FP32, contiguous): W1[256][64] b1[256] W2[256][256] b2[256]
-// W3[256][256] b3[256] W4[64][256] b4[64]; the BF16 block holds the four W's in the same
-// order (biases stay FP32).
+// W3[256][256] b3[256] W4[64][256] b4[64].
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "../../include/kinetix_mi.h"
+#include "kx_mlp_wave.h"
 
 namespace {
 
-constexpr int kIn = 64, kH = 256, kOut = 64, kValidOut = 55;
-constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH;
-constexpr int kW3 = kB2 + kH, kB3 = kW3 + kH * kH, kW4 = kB3 + kH, kB4 = kW4 + kOut * kH;
-constexpr int kParams = kB4 + kOut;
-constexpr int kWb1 = 0, kWb2 = kWb1 + kH * kIn, kWb3 = kWb2 + kH * kH, kWb4 = kWb3 + kH * kH;
-constexpr int kWeightsBf16 = kWb4 + kOut * kH;
+using kxmlp::kIn;
+using kxmlp::kH;
+using kxmlp::kOut;
+using kxmlp::kValidOut;
+using kxmlp::kW1;
+using kxmlp::kB1;
+using kxmlp::kW2;
+using kxmlp::kB2;
+using kxmlp::kW3;
+using kxmlp::kB3;
+using kxmlp::kW4;
+using kxmlp::kB4;
+using kxmlp::kParams;
+constexpr int kWeightsBf16 = kxmlp::kFragTotal;
 
 __device__ __forceinline__ float gelu(float x)
 {
@@ -164,108 +171,57 @@ __global__ void __launch_bounds__(256) adam(int n, float* __restrict__ w, const 
   w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
+// BF16 weights in MFMA fragment order: fragment (ob, kb) of layer l, lane i + 16 g,
+// element e = W_l[16 ob + i][kphi(kb, g, e)] (kx_mlp_wave.h).
 __global__ void __launch_bounds__(256) pack_bf16(const float* __restrict__ p, uint16_t* __restrict__ wb)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kWeightsBf16) return;
-  int src;
-  if (i < kWb2) src = kW1 + i;
-  else if (i < kWb3) src = kW2 + (i - kWb2);
-  else if (i < kWb4) src = kW3 + (i - kWb3);
-  else src = kW4 + (i - kWb4);
-  wb[i] = to_bf16(p[src]);
+  int base, K, w;
+  if (i < kxmlp::kF2)      { base = kxmlp::kF1; K = kIn; w = kW1; }
+  else if (i < kxmlp::kF3) { base = kxmlp::kF2; K = kH;  w = kW2; }
+  else if (i < kxmlp::kF4) { base = kxmlp::kF3; K = kH;  w = kW3; }
+  else                     { base = kxmlp::kF4; K = kH;  w = kW4; }
+  const int local = i - base, e = local & 7, lane = (local >> 3) & 63, frag = local >> 9;
+  const int KB = K / 32, kb = frag % KB, ob = frag / KB;
+  const int row = 16 * ob + (lane & 15), col = kxmlp::kphi(kb, lane >> 4, e);
+  wb[i] = to_bf16(p[w + row * K + col]);
 }
 
-// ---------------------------------------------------------------------------------------
-// Fused BF16-MFMA inference. v_mfma_f32_16x16x16_bf16 lane layout (checked on the device
-// by kx_selftest_mfma_layout): A[i][k] in lane i + 16 (k / 4), element k % 4;
-// B[k][j] in lane j + 16 (k / 4), element k % 4; D[i][j] in element i % 4 of lane j + 16 (i / 4).
+// BF16-MFMA inference of n states (row-major FP32 X and Y, n x 64): 4 waves x 64 states
+// per workgroup; the inputs go through LDS once to reach the MFMA fragment layout.
+constexpr int kXS = 68;       // LDS row stride (bf16) of the staged inputs
+
+__global__ void __launch_bounds__(256, 1) mlp_infer_bf16(const uint16_t* __restrict__ wb,
+                                                         const float* __restrict__ p, int n_states,
+                                                         const float* __restrict__ X, float* __restrict__ Y)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t xs[4][64 * kXS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long s0 = (long long)blockIdx.x * 256 + wv * 64;
+  // stage the wave's 64 input rows as BF16 (each pass reads one 256-B row, coalesced)
+  for (int e = lane; e < 64 * kIn; e += 64) {
+    const int row = e >> 6, col = e & 63;
+    const float x = (s0 + row < n_states) ? X[(s0 + row) * kIn + col] : 0.0f;
+    xs[wv][row * kXS + col] = to_bf16(x);
+  }
+  __syncthreads();
+  kxmlp::f32x4 y[kOut / 16][4];
+  kxmlp::forward<kXS>(wb, p, xs[wv], y, lane);
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) {
+    const long long s = s0 + 16 * sb + j;
+    if (s < n_states) {
+#pragma unroll
+      for (int ob = 0; ob < kOut / 16; ++ob)
+        *(kxmlp::f32x4*)(Y + s * kOut + 16 * ob + 4 * g) = y[ob][sb];
+    }
+  }
+}
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-constexpr int kTile = 128;        // states per workgroup
-constexpr int kHS = 260;          // LDS row stride (bf16) of the activation buffers
-constexpr int kWS = 36;           // LDS row stride (bf16) of a 32-wide weight chunk
-
-template <int KIN, int NOUT, bool LAST>
-__device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ Wg, const float* __restrict__ bias,
-                                          const uint16_t* Hin, uint16_t* Hout, uint16_t* Wc,
-                                          float* __restrict__ out, long long s0, int n_states)
-{
-  constexpr int CPW = NOUT / 4;                  // output columns per wave
-  constexpr int CB = CPW / 16;                   // column blocks per wave
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int li = lane & 15, lg = lane >> 4;
-  floatx4 acc[8][CB];
-#pragma unroll
-  for (int r = 0; r < 8; ++r)
-#pragma unroll
-    for (int c = 0; c < CB; ++c) acc[r][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < KIN; k0 += 32) {
-    // stage W[:, k0:k0+32] (NOUT x 32 bf16) in LDS: 8 bf16 (16 B) per load
-    for (int e = threadIdx.x; e < NOUT * 4; e += 256) {
-      const int row = e >> 2, q = e & 3;
-      const uint4 v = *(const uint4*)(Wg + (size_t)row * KIN + k0 + 8 * q);
-      uint16_t* d = Wc + row * kWS + 8 * q;
-      *(uint2*)d = make_uint2(v.x, v.y);
-      *(uint2*)(d + 4) = make_uint2(v.z, v.w);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < 32; ks += 16) {
-      bf16x4 b[CB];
-#pragma unroll
-      for (int c = 0; c < CB; ++c)
-        b[c] = *(const bf16x4*)(Wc + (wv * CPW + c * 16 + li) * kWS + ks + 4 * lg);
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const bf16x4 a = *(const bf16x4*)(Hin + (r * 16 + li) * kHS + k0 + ks + 4 * lg);
-#pragma unroll
-        for (int c = 0; c < CB; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[c], acc[r][c], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < 8; ++r)
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-      const int col = wv * CPW + c * 16 + li;
-      const float bc = bias[col];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = r * 16 + 4 * lg + i;
-        const float z = acc[r][c][i] + bc;
-        if (LAST) {
-          if (s0 + row < n_states) out[(size_t)(s0 + row) * kOut + col] = z;
-        } else {
-          Hout[row * kHS + col] = to_bf16(gelu(z));
-        }
-      }
-    }
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(256) mlp_infer_bf16(const uint16_t* __restrict__ wb,
-                                                      const float* __restrict__ p, int n_states,
-                                                      const float* __restrict__ X, float* __restrict__ Y)
-{
-  __shared__ __attribute__((aligned(16))) uint16_t H0[kTile * kHS];
-  __shared__ __attribute__((aligned(16))) uint16_t H1[kTile * kHS];
-  __shared__ __attribute__((aligned(16))) uint16_t Wc[kH * kWS];
-  const long long s0 = (long long)blockIdx.x * kTile;
-  for (int e = threadIdx.x; e < kTile * kIn; e += 256) {
-    const int row = e >> 6, col = e & 63;
-    const float x = (s0 + row < n_states) ? X[(size_t)(s0 + row) * kIn + col] : 0.0f;
-    H0[row * kHS + col] = to_bf16(x);
-  }
-  __syncthreads();
-  mlp_layer<kIn, kH, false>(wb + kWb1, p + kB1, H0, H1, Wc, nullptr, s0, n_states);
-  mlp_layer<kH, kH, false>(wb + kWb2, p + kB2, H1, H0, Wc, nullptr, s0, n_states);
-  mlp_layer<kH, kH, false>(wb + kWb3, p + kB3, H0, H1, Wc, nullptr, s0, n_states);
-  mlp_layer<kH, kOut, true>(wb + kWb4, p + kB4, H1, nullptr, Wc, Y, s0, n_states);
-}
 
 // Self-test of the BF16 MFMA layout: D = A B for closed-form 16x16 A and B.
 __global__ void __launch_bounds__(64) bf16_layout_probe(float* __restrict__ d)
@@ -279,6 +235,28 @@ __global__ void __launch_bounds__(64) bf16_layout_probe(float* __restrict__ d)
   }
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
   acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) d[(4 * g + r) * 16 + i] = acc[r];
+}
+
+// Self-test of the v_mfma_f32_16x16x32_bf16 layout the inference path relies on:
+// A[i][k] in element k % 8 of lane i + 16 (k / 8), B[k][j] likewise with j, D as above;
+// small integer entries so that every product and sum is exact.
+__global__ void __launch_bounds__(64) bf16_k32_layout_probe(float* __restrict__ d)
+{
+  const int l = threadIdx.x, i = l & 15, g = l >> 4;
+  kxmlp::u32x4 a, b;
+  for (int h = 0; h < 4; ++h) {
+    uint32_t wa = 0, wbv = 0;
+    for (int q = 0; q < 2; ++q) {
+      const int k = 8 * g + 2 * h + q;
+      wa |= (uint32_t)to_bf16((float)((i * 7 + k * 3) % 17 - 8)) << (16 * q);     // A[i][k]
+      wbv |= (uint32_t)to_bf16((float)((k * 5 + i * 11) % 13 - 6)) << (16 * q);   // B[k][j = i]
+    }
+    a[h] = wa;
+    b[h] = wbv;
+  }
+  kxmlp::f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = kxmlp::mfma(a, b, acc);
   for (int r = 0; r < 4; ++r) d[(4 * g + r) * 16 + i] = acc[r];
 }
 
@@ -297,6 +275,13 @@ int kx_selftest_mfma_bf16_layout(float* d_out, void* stream)
 {
   if (!d_out) return KX_ERR_INVALID_ARG;
   hipLaunchKernelGGL(bf16_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, d_out);
+  return hip_status();
+}
+
+int kx_selftest_mfma_bf16_k32_layout(float* d_out, void* stream)
+{
+  if (!d_out) return KX_ERR_INVALID_ARG;
+  hipLaunchKernelGGL(bf16_k32_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, d_out);
   return hip_status();
 }
 
@@ -376,7 +361,7 @@ int kx_mlp_infer_bf16(const uint16_t* wb, const float* p, int n, const float* X,
 {
   if (!wb || !p || n < 0 || (n > 0 && (!X || !Y))) return KX_ERR_INVALID_ARG;
   if (n == 0) return KX_OK;
-  hipLaunchKernelGGL(mlp_infer_bf16, dim3((n + kTile - 1) / kTile), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(mlp_infer_bf16, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                      wb, p, n, X, Y);
   return hip_status();
 }

@@ -1,0 +1,157 @@
+// Wave-level BF16-MFMA forward pass of the transport-surrogate MLP
+// (64 -> 256 -> 256 -> 256 -> 64, GELU on the hidden layers) for 64 states per wavefront.
+//
+// Layout choices (gfx950, v_mfma_f32_16x16x32_bf16, wave64):
+//  * every layer is computed transposed, Z^T = W H^T: the MFMA's A operand is a 16 x 32
+//    block of weights (rows = output features), B is a 32 x 16 block of activations
+//    (columns = states), D is 16 features x 16 states;
+//  * the D layout of 16x16 MFMAs (row i of column j in element i % 4 of lane j + 16 (i / 4))
+//    is, element for element, the B layout of the next layer (row k of column j in element
+//    k % 8 of lane j + 16 (k / 8)) once the next layer's k index is permuted: element e of
+//    lane group g in k-block kb holds feature kphi(kb, g, e), i.e. 4 features from output
+//    block 2 kb and 4 from block 2 kb + 1. The weights are pre-packed with the same
+//    permutation, so activations never leave registers between layers;
+//  * weights stream from L2 as pre-packed fragments (one fragment = 64 lanes x 8 bf16 =
+//    1 KB, contiguous), each reused for the wave's 4 column blocks of 16 states;
+//  * bias is the accumulator's initial value; GELU (tanh form, written as x * sigmoid(2u))
+//    costs one exp2 and one reciprocal; the results are packed to BF16 with v_cvt_pk_bf16_f32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kxmlp {
+
+constexpr int kIn = 64, kH = 256, kOut = 64, kValidOut = 55;
+// FP32 parameter block: W1[256][64] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
+// W4[64][256] b4[64]
+constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH;
+constexpr int kW3 = kB2 + kH, kB3 = kW3 + kH * kH, kW4 = kB3 + kH, kB4 = kW4 + kOut * kH;
+constexpr int kParams = kB4 + kOut;
+static_assert(kB1 % 4 == 0 && kB2 % 4 == 0 && kB3 % 4 == 0 && kB4 % 4 == 0, "16-B bias loads");
+
+// BF16 fragment block: layer l holds (out/16) x (in/32) fragments of 512 bf16, ordered
+// [output block][k block][lane][8]
+constexpr int kFrag = 512;
+constexpr int kF1 = 0;
+constexpr int kF2 = kF1 + (kH / 16) * (kIn / 32) * kFrag;
+constexpr int kF3 = kF2 + (kH / 16) * (kH / 32) * kFrag;
+constexpr int kF4 = kF3 + (kH / 16) * (kH / 32) * kFrag;
+constexpr int kFragTotal = kF4 + (kOut / 16) * (kH / 32) * kFrag;
+
+// input feature held by element e (0..7) of lane group g (0..3) in k-block kb
+__host__ __device__ constexpr int kphi(int kb, int g, int e)
+{
+  return 32 * kb + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack2(float a, float b)
+{
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh u) = x / (1 + exp(-2u)), u = sqrt(2/pi) (x + 0.044715 x^3);
+// exp(-2u) = exp2(-x (c0 + c1 x^2)) with c0 = 2 sqrt(2/pi) log2(e), c1 = 0.044715 c0.
+__device__ __forceinline__ float gelu_fast(float x)
+{
+  constexpr float c0 = 2.302208198144325f, c1 = 0.10294324037962224f;
+  const float a = x * __builtin_fmaf(c1, x * x, c0);
+  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-a));
+}
+
+// activations of the wave's 64 states for one layer: [k block][state block] fragments
+template <int KB>
+using Act = u32x4[KB][4];
+
+// B fragments of the first layer from LDS: xs holds the wave's 64 states as rows of XS bf16
+// (feature f at column f).
+template <int XS>
+__device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32>& in, int lane)
+{
+  static_assert(XS % 4 == 0, "8-byte LDS reads");
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int kb = 0; kb < kIn / 32; ++kb)
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+      const uint16_t* r = xs + (16 * sb + j) * XS + 32 * kb + 4 * g;
+      const uint2 lo = *(const uint2*)r;           // features kphi(kb, g, 0..3)
+      const uint2 hi = *(const uint2*)(r + 16);    // features kphi(kb, g, 4..7)
+      in[kb][sb] = (u32x4){lo.x, lo.y, hi.x, hi.y};
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+                                             const Act<KB>& in, Act<8>& out, int lane)
+{
+  const int g = lane >> 4;
+#pragma unroll
+  for (int ob = 0; ob < kH / 16; ++ob) {
+    const f32x4 b = *(const f32x4*)(bias + 16 * ob + 4 * g);
+    f32x4 acc[4] = {b, b, b, b};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const u32x4 a = wf[(ob * KB + kb) * 64 + lane];
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a, in[kb][sb], acc[sb]);
+    }
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+      const unsigned lo = pack2(gelu_fast(acc[sb][0]), gelu_fast(acc[sb][1]));
+      const unsigned hi = pack2(gelu_fast(acc[sb][2]), gelu_fast(acc[sb][3]));
+      out[ob >> 1][sb][(ob & 1) * 2] = lo;
+      out[ob >> 1][sb][(ob & 1) * 2 + 1] = hi;
+    }
+  }
+}
+
+// y[ob][sb][e]: output feature 16 ob + 4 (lane / 16) + e of state 16 sb + lane % 16
+__device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+                                             const Act<8>& in, f32x4 (&y)[kOut / 16][4], int lane)
+{
+  const int g = lane >> 4;
+#pragma unroll
+  for (int ob = 0; ob < kOut / 16; ++ob) {
+    const f32x4 b = *(const f32x4*)(bias + 16 * ob + 4 * g);
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) y[ob][sb] = b;
+#pragma unroll
+    for (int kb = 0; kb < kH / 32; ++kb) {
+      const u32x4 a = wf[(ob * (kH / 32) + kb) * 64 + lane];
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) y[ob][sb] = mfma(a, in[kb][sb], y[ob][sb]);
+    }
+  }
+}
+
+// Full forward pass for the wave's 64 states. wf: fragment block (kFragTotal bf16, 16-B
+// aligned); p: FP32 parameter block (biases); xs: LDS rows of the standardised inputs.
+template <int XS>
+__device__ __forceinline__ void forward(const uint16_t* __restrict__ wfrag, const float* __restrict__ p,
+                                        const uint16_t* xs, f32x4 (&y)[kOut / 16][4], int lane)
+{
+  const u32x4* wf = (const u32x4*)wfrag;
+  Act<kIn / 32> x0;
+  load_input<XS>(xs, x0, lane);
+  Act<8> h1, h2;
+  hidden_layer<kIn / 32>(wf + kF1 / 8, p + kB1, x0, h1, lane);
+  hidden_layer<8>(wf + kF2 / 8, p + kB2, h1, h2, lane);
+  hidden_layer<8>(wf + kF3 / 8, p + kB3, h2, h1, lane);
+  output_layer(wf + kF4 / 8, p + kB4, h1, y, lane);
+}
+
+}  // namespace kxmlp

@@ -14,11 +14,15 @@ kernels in kinetix_amd/csrc/kx_mlp.hip) learns the standardised residual:
     state's T and p, from the same pair fits.
 
 Kernels:
+  kx_sur_infer     the inference path in one pass: inputs, baselines and hull flag (one state
+                   per lane), the MLP on BF16 MFMA (64 states per wavefront,
+                   kinetix_amd/csrc/kx_mlp_wave.h), outputs mu, lambda, rho*D_km (and D_km)
+                   in FP64 for states not flagged for the plain-kernel fallback
   kx_sur_features  state -> standardised MLP input (n x 64, FP32), baseline (n x 64) and an
-                   out-of-hull flag per state (input outside the training box)
+                   out-of-hull flag per state (input outside the training box); training
   kx_sur_targets   plain-kernel labels -> standardised training targets (n x 64)
-  kx_sur_outputs   baseline + de-standardised MLP output -> mu, lambda, rho*D_km (and D_km),
-                   FP64, for states not flagged for the plain-kernel fallback
+  kx_sur_outputs   baseline + de-standardised MLP output -> outputs (the composed path
+                   features -> kx_mlp_infer_bf16 -> outputs, a cross-check of kx_sur_infer)
 """
 from __future__ import annotations
 
@@ -70,6 +74,9 @@ def baseline_numpy(t, species, T, p, Y):
     return np.log(mu), np.log(lam), np.log(D)
 
 
+XS = 68            # LDS row stride (bf16) of the staged MLP inputs in the fused kernel
+
+
 def emit(t, info):
     N = int(t['n_species'])
     if N + 2 > IN:
@@ -82,19 +89,76 @@ def emit(t, info):
     rcp_diff = bool(int(t.get('rcp_diff', 0)))
     S = E._Src()
 
-    def poly(c, v='lnT'):
-        return (f'{lit(c[0])} + {lit(c[1])} * {v} + {lit(c[2])} * {v}2 + {lit(c[3])} * {v}3 + '
-                f'{lit(c[4])} * {v}4')
+    def fpoly(c):
+        return (f'{float(c[0])!r}f + {float(c[1])!r}f * lT + {float(c[2])!r}f * lT2 + '
+                f'{float(c[3])!r}f * lT3 + {float(c[4])!r}f * lT4')
 
-    def prologue(pstate_arg=True):
-        S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
-        S('if (id >= n_states) return;')
-        S('const double T = Tref * state[id];')
-        S('const double p = p_state[id];')
+    def state_model():
+        """Per-state code (one state per lane; T, p, id in scope): raw inputs raw[] and the
+        FP32 physics baselines bl[] = (ln mu, ln lambda, ln D_1m .. ln D_Nm)."""
         S('const double lnT = log(T);')
-        S('const double lnT2 = lnT * lnT, lnT3 = lnT2 * lnT, lnT4 = lnT2 * lnT2;')
+        S(f'float raw[{N + 2}];')
+        S('raw[0] = (float)((T - 1400.0) * (1.0 / 1100.0));')
+        S(f'raw[1] = (float)((log(p) - {lit(math.log(101325.0))}) * (1.0 / 1.3));')
+        for k in range(N):
+            S(f'raw[{k + 2}] = (float)state[id + offsetT + {k} * offset];')
+        S('// baselines (FP32): mole fractions, Wilke viscosity via rank-%d factors, conductivity'
+          ' mixing rule, binary diffusion in the bath gas' % RANK)
+        S('const float lT = (float)lnT, lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
+        S('const float sqT = sqrtf((float)T);')
+        for k in range(N):
+            S(f'const float y{k} = fmaxf(0.0f, raw[{k + 2}]) * {float(rcpW[k])!r}f;')
+        S('const float rsum = 1.0f / (' + ' + '.join(f'y{k}' for k in range(N)) + ');')
+        for k in range(N):
+            S(f'const float x{k} = y{k} * rsum;')
+        for k in range(N):
+            S(f'const float e{k} = {fpoly(t["visc"][k])};')
+            S(f'const float ie{k} = 1.0f / e{k};')
+        for m in range(RANK):
+            for a, (U, V) in enumerate(fac):
+                if a == 0:
+                    terms = [f'{float(V[m, j])!r}f * x{j}' for j in range(N)]
+                elif a == 1:
+                    terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j})' for j in range(N)]
+                else:
+                    terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j} * ie{j})' for j in range(N)]
+                S(f'const float v{a}_{m} = ' + ' + '.join(terms) + ';')
+        S('float mu = 0.0f;')
+        for k in range(N):
+            s1 = ' + '.join(f'{float(fac[0][0][k, m])!r}f * v0_{m}' for m in range(RANK))
+            s2 = ' + '.join(f'{float(fac[1][0][k, m])!r}f * v1_{m}' for m in range(RANK))
+            s3 = ' + '.join(f'{float(fac[2][0][k, m])!r}f * v2_{m}' for m in range(RANK))
+            S(f'{{ const float Sk = ({s1}) + 2.0f * e{k} * ({s2}) + e{k} * e{k} * ({s3});')
+            S(f'  mu += x{k} * e{k} * e{k} / Sk; }}')
+        S('float l1 = 0.0f, l2 = 0.0f;')
+        for k in range(N):
+            S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} / l; }}')
+        S(f'float bl[{N + 2}];')
+        S('bl[0] = logf(sqT * mu);')
+        S('bl[1] = logf(sqT * 0.5f * (l1 + 1.0f / l2));')
+        S('const float lT15p = 1.5f * lT - (float)log(p);')
+        for k in range(N):
+            d = t['diff'][k, pr[k]]
+            expr = f'-logf({fpoly(d)})' if rcp_diff else f'logf({fpoly(d)})'
+            S(f'bl[{k + 2}] = lT15p + {expr};')
 
-    # ---------------- features
+    def hull_and_inputs(store):
+        """out = 1 if any raw input lies outside the hull box (or is NaN); store(i, expr)
+        receives the standardised input i."""
+        S('int out = 0;')
+        S('#pragma unroll')
+        S(f'for (int i = 0; i < {N + 2}; ++i) {{')
+        S(f'  out |= (raw[i] < norm[{2 * IN} + i]) | (raw[i] > norm[{3 * IN} + i]) | (raw[i] != raw[i]);')
+        S('  ' + store('i', f'(raw[i] - norm[i]) / norm[{IN} + i]'))
+        S('}')
+
+    def rcp_mbar():
+        S('double rcpMbar = 0.0;')
+        for k in range(N):
+            S(f'rcpMbar += fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+
+    S('#include "kx_mlp_wave.h"', 0)
+    # ---------------- features (training and the composed inference path)
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
     S('kx_sur_features(const int n_states, const long long offsetT, const long long offset,', 0)
     S('                const double* __restrict__ p_state, const double* __restrict__ state,', 0)
@@ -102,65 +166,18 @@ def emit(t, info):
     S('                float* __restrict__ base, int* __restrict__ flags, const double Tref)', 0)
     S('{', 0)
     S(E.POOL_MARK)
-    prologue()
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double T = Tref * state[id];')
+    S('const double p = p_state[id];')
     S('// norm: in_mean[64], in_std[64], hull_lo[64], hull_hi[64] (raw input units)')
-    S('float raw[%d];' % (N + 2))
-    S('raw[0] = (float)((T - 1400.0) * (1.0 / 1100.0));')
-    S(f'raw[1] = (float)((log(p) - {lit(math.log(101325.0))}) * (1.0 / 1.3));')
-    for k in range(N):
-        S(f'raw[{k + 2}] = (float)state[id + offsetT + {k} * offset];')
-    S('int out = 0;')
+    state_model()
     S(f'float* xr = xin + id * {IN};')
-    S(f'#pragma unroll')
-    S(f'for (int i = 0; i < {N + 2}; ++i) {{')
-    S(f'  out |= (raw[i] < norm[{2 * IN} + i]) | (raw[i] > norm[{3 * IN} + i]) | (raw[i] != raw[i]);')
-    S(f'  xr[i] = (raw[i] - norm[i]) / norm[{IN} + i];')
-    S('}')
+    hull_and_inputs(lambda i, e: f'xr[{i}] = {e};')
     S(f'for (int i = {N + 2}; i < {IN}; ++i) xr[i] = 0.0f;')
     S('flags[id] = out;')
-    S('// baselines (FP32): mole fractions, Wilke viscosity via rank-%d factors, conductivity'
-      ' mixing rule, binary diffusion in the bath gas' % RANK)
-    S('const float lT = (float)lnT, lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
-    S('const float sqT = sqrtf((float)T);')
-    for k in range(N):
-        S(f'const float y{k} = fmaxf(0.0f, raw[{k + 2}]) * {float(rcpW[k])!r}f;')
-    S('const float rsum = 1.0f / (' + ' + '.join(f'y{k}' for k in range(N)) + ');')
-    for k in range(N):
-        S(f'const float x{k} = y{k} * rsum;')
-
-    def fpoly(c):
-        return (f'{float(c[0])!r}f + {float(c[1])!r}f * lT + {float(c[2])!r}f * lT2 + '
-                f'{float(c[3])!r}f * lT3 + {float(c[4])!r}f * lT4')
-    for k in range(N):
-        S(f'const float e{k} = {fpoly(t["visc"][k])};')
-        S(f'const float ie{k} = 1.0f / e{k};')
-    for m in range(RANK):
-        for a, (U, V) in enumerate(fac):
-            if a == 0:
-                terms = [f'{float(V[m, j])!r}f * x{j}' for j in range(N)]
-            elif a == 1:
-                terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j})' for j in range(N)]
-            else:
-                terms = [f'{float(V[m, j])!r}f * (x{j} * ie{j} * ie{j})' for j in range(N)]
-            S(f'const float v{a}_{m} = ' + ' + '.join(terms) + ';')
-    S('float mu = 0.0f;')
-    for k in range(N):
-        s1 = ' + '.join(f'{float(fac[0][0][k, m])!r}f * v0_{m}' for m in range(RANK))
-        s2 = ' + '.join(f'{float(fac[1][0][k, m])!r}f * v1_{m}' for m in range(RANK))
-        s3 = ' + '.join(f'{float(fac[2][0][k, m])!r}f * v2_{m}' for m in range(RANK))
-        S(f'{{ const float Sk = ({s1}) + 2.0f * e{k} * ({s2}) + e{k} * e{k} * ({s3});')
-        S(f'  mu += x{k} * e{k} * e{k} / Sk; }}')
-    S('float l1 = 0.0f, l2 = 0.0f;')
-    for k in range(N):
-        S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} / l; }}')
     S(f'float* br = base + id * {IN};')
-    S('br[0] = logf(sqT * mu);')
-    S('br[1] = logf(sqT * 0.5f * (l1 + 1.0f / l2));')
-    S('const float lT15p = 1.5f * lT - (float)log(p);')
-    for k in range(N):
-        d = t['diff'][k, pr[k]]
-        expr = f'-logf({fpoly(d)})' if rcp_diff else f'logf({fpoly(d)})'
-        S(f'br[{k + 2}] = lT15p + {expr};')
+    S(f'for (int i = 0; i < {N + 2}; ++i) br[i] = bl[i];')
     S(f'for (int i = {N + 2}; i < {IN}; ++i) br[i] = 0.0f;')
     S('}', 0)
 
@@ -183,7 +200,7 @@ def emit(t, info):
     S(f'for (int i = {N + 2}; i < {IN}; ++i) tr[i] = 0.0f;')
     S('}', 0)
 
-    # ---------------- outputs
+    # ---------------- outputs of the composed path
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
     S('kx_sur_outputs(const int n_states, const long long offsetT, const long long offset,', 0)
     S('               const double* __restrict__ p_state, const double* __restrict__ state,', 0)
@@ -200,14 +217,82 @@ def emit(t, info):
     S(f'const float* rr = resid + id * {IN};')
     S(f'viscosity[id] = exp((double)br[0] + (double)ostat[0] + (double)ostat[{IN}] * (double)rr[0]);')
     S(f'conductivity[id] = exp((double)br[1] + (double)ostat[1] + (double)ostat[{IN + 1}] * (double)rr[1]);')
-    S('double rcpMbar = 0.0;')
-    for k in range(N):
-        S(f'rcpMbar += fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    rcp_mbar()
     S(f'const double rho = p * {lit(1.0 / float(t["R"]))} / (T * rcpMbar);')
     S(f'for (int k = 0; k < {N}; ++k) {{')
     S(f'  const double D = exp((double)br[k + 2] + (double)ostat[k + 2] + (double)ostat[{IN} + k + 2] * (double)rr[k + 2]);')
     S('  rhoD[k * offset + id] = rho * D;')
     S('  if (Dkm) Dkm[k * offset + id] = D;')
+    S('}')
+    S('}', 0)
+
+    # ---------------- fused inference: features + MLP (BF16 MFMA) + outputs in one pass
+    BS = N + 2 + (1 - (N + 2) % 2)                 # odd LDS row stride of the baselines
+    S('// One state per lane for the physics part, 64 states per wavefront for the MLP')
+    S('// (kx_mlp_wave.h); LDS carries the inputs and baselines between the two layouts.')
+    S('// States flagged out of hull (or beyond n_states) get no output here: the caller')
+    S('// routes them to the plain kernel.')
+    S('extern "C" __global__ void __launch_bounds__(256, 1)', 0)
+    S('kx_sur_infer(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('             const double* __restrict__ p_state, const double* __restrict__ state,', 0)
+    S('             const float* __restrict__ norm, const float* __restrict__ ostat,', 0)
+    S('             const uint16_t* __restrict__ wfrag, const float* __restrict__ params,', 0)
+    S('             int* __restrict__ flags, double* __restrict__ viscosity,', 0)
+    S('             double* __restrict__ conductivity, double* __restrict__ rhoD,', 0)
+    S('             double* __restrict__ Dkm, const double Tref)', 0)
+    S('{', 0)
+    S(E.POOL_MARK)
+    S(f'__shared__ __attribute__((aligned(16))) uint16_t xs[4][64 * {XS}];')
+    S(f'__shared__ float bs[4][64 * {BS}];')
+    S('__shared__ double rs[4][64];')
+    S('__shared__ int fs[4][64];')
+    S('const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;')
+    S('const long long id = (long long)blockIdx.x * 256 + threadIdx.x;')
+    S(f'uint16_t* xr = xs[wv] + lane * {XS};')
+    S(f'float* br = bs[wv] + lane * {BS};')
+    S('if (id < n_states) {')
+    S('const double T = Tref * state[id];')
+    S('const double p = p_state[id];')
+    state_model()
+    S(f'float xn[{IN}];')
+    hull_and_inputs(lambda i, e: f'xn[{i}] = {e};')
+    S(f'for (int i = {N + 2}; i < {IN}; ++i) xn[i] = 0.0f;')
+    S(f'for (int i = 0; i < {IN}; i += 2)')
+    S('  *(unsigned*)(xr + i) = kxmlp::pack2(xn[i], xn[i + 1]);')
+    S(f'for (int i = 0; i < {N + 2}; ++i) br[i] = bl[i] + ostat[i];')
+    rcp_mbar()
+    S(f'rs[wv][lane] = p * {lit(1.0 / float(t["R"]))} / (T * rcpMbar);')
+    S('fs[wv][lane] = out;')
+    S('flags[id] = out;')
+    S('} else {')
+    S(f'for (int i = 0; i < {IN}; i += 2) *(unsigned*)(xr + i) = 0u;')
+    S('fs[wv][lane] = 1;')
+    S('}')
+    S('__syncthreads();')
+    S('kxmlp::f32x4 y[kxmlp::kOut / 16][4];')
+    S(f'kxmlp::forward<{XS}>(wfrag, params, xs[wv], y, lane);')
+    S('const int j = lane & 15, g = lane >> 4;')
+    S('#pragma unroll')
+    S('for (int sb = 0; sb < 4; ++sb) {')
+    S('  const int s = 16 * sb + j;')
+    S('  const long long sid = (long long)blockIdx.x * 256 + wv * 64 + s;')
+    S('  if (fs[wv][s]) continue;')
+    S('  const double rho = rs[wv][s];')
+    S('  #pragma unroll')
+    S('  for (int ob = 0; ob < kxmlp::kOut / 16; ++ob)')
+    S('  #pragma unroll')
+    S('  for (int e = 0; e < 4; ++e) {')
+    S('    const int f = 16 * ob + 4 * g + e;')
+    S(f'    if (f >= {N + 2}) continue;')
+    S(f'    const float v = bs[wv][s * {BS} + f] + ostat[{IN} + f] * y[ob][sb][e];')
+    S('    const double D = (double)__builtin_amdgcn_exp2f(v * 1.4426950408889634f);')
+    S('    if (f == 0) viscosity[sid] = D;')
+    S('    else if (f == 1) conductivity[sid] = D;')
+    S('    else {')
+    S('      rhoD[(f - 2) * offset + sid] = rho * D;')
+    S('      if (Dkm) Dkm[(f - 2) * offset + sid] = D;')
+    S('    }')
+    S('  }')
     S('}')
     S('}', 0)
     return S.text()

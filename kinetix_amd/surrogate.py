@@ -38,6 +38,7 @@ def _bind():
     L.kx_sur_features.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_sur_targets.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_sur_outputs.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_sur_infer.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.kx_mlp_forward.argtypes = [vp, ci, vp, vp, vp]
     L.kx_mlp_backward.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
@@ -224,12 +225,10 @@ class Surrogate:
         N, dev = self.N, self.dev
         if n == 0:
             return {'n': 0, 'fallback': 0, 'fallback_fraction': 0.0}
-        xin, base, flags = self.features(n, offsetT, offset, d_p, d_state)
-        resid = torch.empty((n, IN), dtype=torch.float32, device=dev)
-        _check(self.L.kx_mlp_infer_bf16(_p(self.wbf16), _p(self.params), n, _p(xin), _p(resid), _s()))
-        _check(self.L.kx_sur_outputs(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state), _p(base),
-                                     _p(resid), _p(self.ostat), _p(flags), _p(d_visc), _p(d_cond),
-                                     _p(d_rhoD), _p(d_Dkm), _s()))
+        flags = self._flags(n)
+        _check(self.L.kx_sur_infer(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state), _p(self.norm),
+                                   _p(self.ostat), _p(self.wbf16), _p(self.params), _p(flags),
+                                   _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm), _s()))
         out_of_hull = int(flags.sum())
         # audit: every audit_stride-th state through the plain kernel
         aidx = torch.arange(audit_offset % max(audit_stride, 1), n, max(audit_stride, 1), device=dev)
@@ -264,6 +263,24 @@ class Surrogate:
         return {'n': n, 'out_of_hull': out_of_hull, 'audited': audited, 'audit_failed': n_fail,
                 'whole_call_fallback': bool(whole), 'fallback': n_fb,
                 'fallback_fraction': n_fb / n}
+
+    def infer_composed(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None):
+        """The same model as three kernels (features -> kx_mlp_infer_bf16 -> outputs) with no
+        audit or fallback: a cross-check of the fused kernel. Returns the flags."""
+        import torch
+        xin, base, flags = self.features(n, offsetT, offset, d_p, d_state)
+        resid = torch.empty((n, IN), dtype=torch.float32, device=self.dev)
+        _check(self.L.kx_mlp_infer_bf16(_p(self.wbf16), _p(self.params), n, _p(xin), _p(resid), _s()))
+        _check(self.L.kx_sur_outputs(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state), _p(base),
+                                     _p(resid), _p(self.ostat), _p(flags), _p(d_visc), _p(d_cond),
+                                     _p(d_rhoD), _p(d_Dkm), _s()))
+        return flags
+
+    def _flags(self, n):
+        import torch
+        if getattr(self, '_flag_buf', None) is None or self._flag_buf.numel() < n:
+            self._flag_buf = torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
+        return self._flag_buf[:n]
 
     def _gather(self, idx, offsetT, offset, d_state, d_p):
         import torch

@@ -184,3 +184,29 @@ def test_corrupted_weights_are_caught(trained, gri):
             assert torch.equal(a, b)
     finally:
         sur.set_model(good.cpu().numpy(), sur.norm.cpu().numpy(), sur.ostat.cpu().numpy(), sur.meta)
+
+
+@pytest.mark.parametrize('n', [5000, 64, 1])
+def test_fused_kernel_matches_composed_path(trained, gri, n):
+    """kx_sur_infer (one kernel) against features -> kx_mlp_infer_bf16 -> outputs: the same
+    model and BF16 arithmetic; they differ only in the final exp (FP32 vs FP64)."""
+    info, t = gri
+    kx, sur, _ = trained
+    N = kx.nSpecies()
+    s, p = states.sample_transport_set(n, info['species'], t['W'], seed=11 + n)
+    if n > 1:
+        s[0, 1] = 3500.0                                        # one out-of-hull state
+    ds = torch.from_numpy(np.ascontiguousarray(s)).to(dev).reshape(-1)
+    dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+    z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
+    a = [z(n), z(n), z(N * n), z(N * n)]
+    b = [z(n), z(n), z(N * n), z(N * n)]
+    res = sur.infer(n, n, n, dp, ds, *a, audit_stride=0)
+    fl = sur.infer_composed(n, n, n, dp, ds, *b)
+    torch.cuda.synchronize()
+    ok = (fl == 0)
+    assert res['out_of_hull'] == int((~ok).sum())
+    for x, y in zip(a, b):
+        x_, y_ = x.view(-1, n)[:, ok], y.view(-1, n)[:, ok]
+        assert torch.all(torch.isfinite(x_))
+        assert float(((x_ - y_).abs() / y_.abs()).max()) <= 1e-5
