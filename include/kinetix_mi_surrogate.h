@@ -10,6 +10,7 @@
 #ifndef KINETIX_MI_SURROGATE_H
 #define KINETIX_MI_SURROGATE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "kinetix_mi.h"
@@ -43,13 +44,44 @@ int kx_sur_outputs(kx_mech_t m, int n_states, int offsetT, int offset, const dou
 /* The whole inference path in one kernel: inputs, baselines and hull flags (flags[n], as in
  * kx_sur_features), the MLP on BF16 MFMA (weights from kx_mlp_pack_bf16, biases from the
  * FP32 parameter block; both 16-byte aligned) and the outputs of kx_sur_outputs, for every
- * state whose flag is 0. Flagged states are left untouched for the plain-kernel fallback.
- * d_Dkm may be NULL. This is what the audited inference (kinetix_amd/surrogate.py) calls. */
+ * state whose flag is 0. Flagged states are left untouched; if d_fb_queue is not NULL their
+ * indices are appended to it at d_fb_queue[atomicAdd(d_fb_count, 1)]. d_Dkm may be NULL. */
 int kx_sur_infer(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
                  const double *d_state, const float *d_norm, const float *d_ostat,
                  const uint16_t *d_weights_bf16, const float *d_params, int *d_flags,
                  double *d_viscosity, double *d_conductivity, double *d_rhoD, double *d_Dkm,
-                 void *stream);
+                 int *d_fb_queue, int *d_fb_count, void *stream);
+
+/* Fallback statistics of one kx_sur_transport call. */
+typedef struct {
+  int n_states;
+  int out_of_hull;          /* inputs outside the training box: plain kernel */
+  int audited;              /* audit samples (computed by the plain kernel, which they keep) */
+  int audit_failed;         /* audit samples whose mu or lambda was off by more than audit_tol */
+  int whole_call_fallback;  /* 1: audit failures above max_audit_fail, plain kernel everywhere */
+  int fallback;             /* out_of_hull + audit_failed, or n_states after a whole-call fallback */
+} kx_sur_stats;
+
+/* Device workspace (bytes) kx_sur_transport needs for n_states states and this audit stride
+ * (with_dkm: D_km requested). Proportional to n_states; no allocation happens in the call. */
+size_t kx_sur_workspace_bytes(kx_mech_t m, int n_states, int audit_stride, int with_dkm);
+
+/* Audited surrogate transport, the drop-in beside kx_mixture_avg_transport_pstate:
+ *  1. kx_sur_infer on every state (out-of-hull states queued);
+ *  2. the plain FP64 kernel on every audit_stride-th state (from audit_offset % audit_stride;
+ *     audit_stride <= 0: no audit) and on the queued states, from a compact copy, and its
+ *     results written over theirs (audit samples are compared first: relative error of mu
+ *     or lambda above audit_tol, or NaN, counts as a failure);
+ *  3. if failures exceed max_audit_fail * audited, the plain kernel for all states.
+ * Returns after the stream has drained (it reads the counters); stats may be NULL.
+ * Outputs as kx_mixture_avg_transport_pstate (rows k * offset); d_Dkm may be NULL. */
+int kx_sur_transport(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
+                     const double *d_state, const float *d_norm, const float *d_ostat,
+                     const uint16_t *d_weights_bf16, const float *d_params, int audit_stride,
+                     int audit_offset, double audit_tol, double max_audit_fail,
+                     double *d_viscosity, double *d_conductivity, double *d_rhoD, double *d_Dkm,
+                     void *d_workspace, size_t workspace_bytes, kx_sur_stats *stats,
+                     void *stream);
 
 /* ---- mechanism-independent MLP kernels ---- */
 

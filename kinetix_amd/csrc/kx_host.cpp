@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +20,7 @@
 #include <vector>
 
 #include "../../include/kinetix_mi.h"
+#include "../../include/kinetix_mi_surrogate.h"
 
 namespace {
 
@@ -132,7 +135,8 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
   }
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
-                        "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer"}) {
+                        "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
+                        "kx_sur_queue_init", "kx_sur_gather", "kx_sur_merge"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
     else (void)hipGetLastError();
@@ -216,7 +220,8 @@ int kx_production_rates_mixed_valu(kx_mech_t h, int n, int offsetT, int offset, 
 
 static int transport_impl(kx_mech_t h, int n, int offsetT, int offset, double pressure,
                           const double* d_p, const double* d_state, double* d_visc,
-                          double* d_cond, double* d_rhoD, double* d_Dkm, void* stream)
+                          double* d_cond, double* d_rhoD, double* d_Dkm, void* stream,
+                          const int* n_dev = nullptr)
 {
   Mech* m = get(h);
   int rc = check_layout(m, n, offsetT, offset, d_state, d_rhoD);
@@ -228,10 +233,10 @@ static int transport_impl(kx_mech_t h, int n, int offsetT, int offset, double pr
   if (d_Dkm) {
     // per-state pressure, D_km output (conductivity, viscosity and rho*D do not depend on p)
     const double* p = d_p;
-    void* args[] = {&n, &oT, &o, &p, &s, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
+    void* args[] = {&n, &oT, &o, &p, &s, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref, &n_dev};
     return launch(m, "kx_bk2_fp64_pstate", n, stream, args);
   }
-  void* args[] = {&n, &oT, &o, &pressure, &s, &d_cond, &d_visc, &d_rhoD, &Tref};
+  void* args[] = {&n, &oT, &o, &pressure, &s, &d_cond, &d_visc, &d_rhoD, &Tref, &n_dev};
   return launch(m, "kx_bk2_fp64", n, stream, args);
 }
 
@@ -318,7 +323,8 @@ int kx_sur_outputs(kx_mech_t h, int n, int offsetT, int offset, const double* d_
 int kx_sur_infer(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
                  const double* d_state, const float* d_norm, const float* d_ostat,
                  const uint16_t* d_wfrag, const float* d_params, int* d_flags, double* d_visc,
-                 double* d_cond, double* d_rhoD, double* d_Dkm, void* stream)
+                 double* d_cond, double* d_rhoD, double* d_Dkm, int* d_fb_queue, int* d_fb_count,
+                 void* stream)
 {
   Mech* m = get(h);
   int rc = check_layout(m, n, offsetT, offset, d_state, d_p);
@@ -328,11 +334,143 @@ int kx_sur_infer(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
     return fail(KX_ERR_INVALID_ARG, "null device pointer");
   if ((reinterpret_cast<uintptr_t>(d_wfrag) | reinterpret_cast<uintptr_t>(d_params)) & 15)
     return fail(KX_ERR_INVALID_ARG, "weights and parameters must be 16-byte aligned");
+  if ((d_fb_queue == nullptr) != (d_fb_count == nullptr))
+    return fail(KX_ERR_INVALID_ARG, "fallback queue and its counter go together");
   long long oT = offsetT, o = offset;
   double Tref = m->ref_T;
   void* args[] = {&n, &oT, &o, &d_p, &d_state, &d_norm, &d_ostat, &d_wfrag, &d_params, &d_flags,
-                  &d_visc, &d_cond, &d_rhoD, &d_Dkm, &Tref};
+                  &d_visc, &d_cond, &d_rhoD, &d_Dkm, &Tref, &d_fb_queue, &d_fb_count};
   return launch(m, "kx_sur_infer", n, stream, args);
+}
+
+// ---- audited surrogate transport: one call, caller-provided workspace, no allocation
+namespace {
+
+struct SurPlan {
+  long long n_audit, audit_max, cap, slots;
+  size_t counters, flags, queue, cstate, cp, cvisc, ccond, crhoD, cDkm, total;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+SurPlan sur_plan(int n, int N, int audit_stride, int audit_offset, int with_dkm)
+{
+  SurPlan P{};
+  const long long stride = audit_stride > 0 ? audit_stride : 0;
+  const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
+  P.audit_max = stride ? (n + stride - 1) / stride : 0;
+  P.n_audit = stride && n > a0 ? (n - a0 + stride - 1) / stride : 0;
+  // out-of-hull states handled per compact pass: 1/64 of the call (at least 4096)
+  P.cap = std::min<long long>(n, std::max<long long>(4096, n / 64));
+  P.slots = P.audit_max + P.cap;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
+  P.counters = take(4 * sizeof(int));
+  P.flags = take((size_t)n * sizeof(int));
+  P.queue = take((size_t)(P.audit_max + n) * sizeof(int));
+  P.cstate = take((size_t)(N + 1) * P.slots * sizeof(double));
+  P.cp = take((size_t)P.slots * sizeof(double));
+  P.cvisc = take((size_t)P.slots * sizeof(double));
+  P.ccond = take((size_t)P.slots * sizeof(double));
+  P.crhoD = take((size_t)N * P.slots * sizeof(double));
+  P.cDkm = with_dkm ? take((size_t)N * P.slots * sizeof(double)) : 0;
+  P.total = o;
+  return P;
+}
+
+}  // namespace
+
+size_t kx_sur_workspace_bytes(kx_mech_t h, int n, int audit_stride, int with_dkm)
+{
+  Mech* m = get(h);
+  if (!m || n < 0) return 0;
+  return sur_plan(n, m->n_species, audit_stride, 0, with_dkm).total;
+}
+
+int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
+                     const double* d_state, const float* d_norm, const float* d_ostat,
+                     const uint16_t* d_wfrag, const float* d_params, int audit_stride,
+                     int audit_offset, double audit_tol, double max_audit_fail, double* d_visc,
+                     double* d_cond, double* d_rhoD, double* d_Dkm, void* d_work,
+                     size_t work_bytes, kx_sur_stats* stats, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_p);
+  if (rc) return rc;
+  if (n > 0 && (!d_norm || !d_ostat || !d_wfrag || !d_params || !d_visc || !d_cond || !d_rhoD))
+    return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  if (n > 0 && !d_work) return fail(KX_ERR_INVALID_ARG, "null workspace");
+  const int N = m->n_species;
+  const SurPlan P = sur_plan(n, N, audit_stride, audit_offset, d_Dkm != nullptr);
+  if (work_bytes < P.total)
+    return fail(KX_ERR_TOO_LARGE, "workspace smaller than kx_sur_workspace_bytes()");
+  if (P.slots > INT32_MAX / (N + 1)) return fail(KX_ERR_TOO_LARGE, "too many states for one call");
+  kx_sur_stats st{};
+  st.n_states = n;
+  if (n == 0) { if (stats) *stats = st; return KX_OK; }
+  char* w = static_cast<char*>(d_work);
+  int* counters = reinterpret_cast<int*>(w + P.counters);
+  int* flags = reinterpret_cast<int*>(w + P.flags);
+  int* queue = reinterpret_cast<int*>(w + P.queue);
+  double* cstate = reinterpret_cast<double*>(w + P.cstate);
+  double* cp = reinterpret_cast<double*>(w + P.cp);
+  double* cvisc = reinterpret_cast<double*>(w + P.cvisc);
+  double* ccond = reinterpret_cast<double*>(w + P.ccond);
+  double* crhoD = reinterpret_cast<double*>(w + P.crhoD);
+  double* cDkm = d_Dkm ? reinterpret_cast<double*>(w + P.cDkm) : nullptr;
+  const long long stride = audit_stride > 0 ? audit_stride : 0;
+  const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
+  int n_audit = (int)P.n_audit, slots = (int)P.slots, cap = (int)P.cap;
+
+  // 1. audit ids and counters
+  {
+    void* args[] = {&n_audit, (void*)&a0, (void*)&stride, &queue, &counters};
+    if ((rc = launch(m, "kx_sur_queue_init", std::max(n_audit, 4), stream, args))) return rc;
+  }
+  // 2. surrogate for every state; out-of-hull ids appended after the audit ids
+  int* fbq = queue + n_audit;
+  if ((rc = kx_sur_infer(h, n, offsetT, offset, d_p, d_state, d_norm, d_ostat, d_wfrag, d_params,
+                         flags, d_visc, d_cond, d_rhoD, d_Dkm, fbq, counters, stream)))
+    return rc;
+  // 3. plain kernel on the audit samples + queued out-of-hull states, then merge
+  auto compact_pass = [&](int n_aud, int* q, int skip) -> int {
+    long long oT = offsetT, o = offset;
+    {
+      void* args[] = {&slots, &n_aud, &cap, &skip, &q, &counters, &oT, &o, (void*)&d_p,
+                      (void*)&d_state, &cstate, &cp};
+      if (int r = launch(m, "kx_sur_gather", slots, stream, args)) return r;
+    }
+    if (int r = transport_impl(h, slots, slots, slots, 1.0, cp, cstate, cvisc, ccond, crhoD, cDkm,
+                               stream, counters + 2))
+      return r;
+    long long oo = offset;
+    void* args[] = {&slots, &n_aud, &q, &counters, &flags, &oo, &audit_tol, &cvisc, &ccond,
+                    &crhoD, &cDkm, &d_visc, &d_cond, &d_rhoD, &d_Dkm};
+    return launch(m, "kx_sur_merge", slots, stream, args);
+  };
+  if ((rc = compact_pass(n_audit, queue, 0))) return rc;
+  int host_counters[4] = {0, 0, 0, 0};
+  KX_HIP(hipMemcpyAsync(host_counters, counters, sizeof(host_counters), hipMemcpyDeviceToHost,
+                        (hipStream_t)stream));
+  KX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  st.out_of_hull = host_counters[0];
+  st.audited = n_audit;
+  st.audit_failed = host_counters[1];
+  if (n_audit > 0 && host_counters[1] > max_audit_fail * n_audit) {
+    // the model disagrees with the plain kernel on too many samples: plain kernel for all
+    st.whole_call_fallback = 1;
+    st.fallback = n;
+    if ((rc = transport_impl(h, n, offsetT, offset, 1.0, d_p, d_state, d_visc, d_cond, d_rhoD,
+                             d_Dkm, stream)))
+      return rc;
+  } else {
+    // out-of-hull states beyond the first pass's capacity, cap at a time
+    for (int done = cap; done < st.out_of_hull; done += cap)
+      if ((rc = compact_pass(0, queue + n_audit + done, done))) return rc;
+    st.fallback = st.out_of_hull + st.audit_failed;
+  }
+  if (stats) *stats = st;
+  return KX_OK;
 }
 
 int kx_n_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_species : KX_ERR_NOT_INITIALIZED; }

@@ -12,7 +12,8 @@
 //    block 2 kb and 4 from block 2 kb + 1. The weights are pre-packed with the same
 //    permutation, so activations never leave registers between layers;
 //  * weights stream from L2 as pre-packed fragments (one fragment = 64 lanes x 8 bf16 =
-//    1 KB, contiguous), each reused for the wave's 4 column blocks of 16 states;
+//    1 KB, contiguous), each reused for the wave's 4 column blocks of 16 states, requested
+//    one output block ahead of their use;
 //  * bias is the accumulator's initial value; GELU (tanh form, written as x * sigmoid(2u))
 //    costs one exp2 and one reciprocal; the results are packed to BF16 with v_cvt_pk_bf16_f32.
 #pragma once
@@ -94,21 +95,32 @@ __device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32>& in
     }
 }
 
+// The KB fragments and the bias of output block ob + 1 are requested before the MFMAs of
+// block ob (bias first: vmcnt retires loads in order, so no wait for the current block's
+// operands can cover the next block's), so their L2 latency overlaps a block's worth of
+// MFMA and GELU work.
 template <int KB>
 __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
                                              const Act<KB>& in, Act<8>& out, int lane)
 {
   const int g = lane >> 4;
+  u32x4 a_cur[KB], a_nxt[KB];
+  f32x4 b_cur = *(const f32x4*)(bias + 4 * g), b_nxt;
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) a_cur[kb] = wf[kb * 64 + lane];
 #pragma unroll
   for (int ob = 0; ob < kH / 16; ++ob) {
-    const f32x4 b = *(const f32x4*)(bias + 16 * ob + 4 * g);
-    f32x4 acc[4] = {b, b, b, b};
+    if (ob + 1 < kH / 16) {
+      b_nxt = *(const f32x4*)(bias + 16 * (ob + 1) + 4 * g);
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const u32x4 a = wf[(ob * KB + kb) * 64 + lane];
-#pragma unroll
-      for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a, in[kb][sb], acc[sb]);
+      for (int kb = 0; kb < KB; ++kb) a_nxt[kb] = wf[((ob + 1) * KB + kb) * 64 + lane];
     }
+    __builtin_amdgcn_sched_barrier(0);      // keep the requests here, ahead of the MFMAs
+    f32x4 acc[4] = {b_cur, b_cur, b_cur, b_cur};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a_cur[kb], in[kb][sb], acc[sb]);
 #pragma unroll
     for (int sb = 0; sb < 4; ++sb) {
       const unsigned lo = pack2(gelu_fast(acc[sb][0]), gelu_fast(acc[sb][1]));
@@ -116,6 +128,9 @@ __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const
       out[ob >> 1][sb][(ob & 1) * 2] = lo;
       out[ob >> 1][sb][(ob & 1) * 2 + 1] = hi;
     }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) a_cur[kb] = a_nxt[kb];
+    b_cur = b_nxt;
   }
 }
 
@@ -124,17 +139,27 @@ __device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const
                                              const Act<8>& in, f32x4 (&y)[kOut / 16][4], int lane)
 {
   const int g = lane >> 4;
+  u32x4 a_cur[kH / 32], a_nxt[kH / 32];
+  f32x4 b_cur = *(const f32x4*)(bias + 4 * g), b_nxt;
+#pragma unroll
+  for (int kb = 0; kb < kH / 32; ++kb) a_cur[kb] = wf[kb * 64 + lane];
 #pragma unroll
   for (int ob = 0; ob < kOut / 16; ++ob) {
-    const f32x4 b = *(const f32x4*)(bias + 16 * ob + 4 * g);
+    if (ob + 1 < kOut / 16) {
+      b_nxt = *(const f32x4*)(bias + 16 * (ob + 1) + 4 * g);
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb) y[ob][sb] = b;
-#pragma unroll
-    for (int kb = 0; kb < kH / 32; ++kb) {
-      const u32x4 a = wf[(ob * (kH / 32) + kb) * 64 + lane];
-#pragma unroll
-      for (int sb = 0; sb < 4; ++sb) y[ob][sb] = mfma(a, in[kb][sb], y[ob][sb]);
+      for (int kb = 0; kb < kH / 32; ++kb) a_nxt[kb] = wf[((ob + 1) * (kH / 32) + kb) * 64 + lane];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) y[ob][sb] = b_cur;
+#pragma unroll
+    for (int kb = 0; kb < kH / 32; ++kb)
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) y[ob][sb] = mfma(a_cur[kb], in[kb][sb], y[ob][sb]);
+#pragma unroll
+    for (int kb = 0; kb < kH / 32; ++kb) a_cur[kb] = a_nxt[kb];
+    b_cur = b_nxt;
   }
 }
 

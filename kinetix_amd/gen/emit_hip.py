@@ -627,13 +627,15 @@ def _emit_bk2_fp64(t, info, pstate=False):
     S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
     S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
     if pstate:
-        S('            double* __restrict__ Dkm, const double Tref)', 0)
+        S('            double* __restrict__ Dkm, const double Tref,', 0)
     else:
-        S('            const double Tref)', 0)
+        S('            const double Tref,', 0)
+    S('            const int* __restrict__ n_dev)', 0)
     S('{', 0)
     S(POOL_MARK)
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
-    S('if (id >= n_states) return;')
+    S('// n_dev (optional): a state count known only on the device (audit/fallback queues)')
+    S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
     S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
     S('const double T = Tref * state[id];')
     S('const double lnT = log(T);')

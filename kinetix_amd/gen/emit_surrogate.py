@@ -96,24 +96,27 @@ def emit(t, info):
     def state_model():
         """Per-state code (one state per lane; T, p, id in scope): raw inputs raw[] and the
         FP32 physics baselines bl[] = (ln mu, ln lambda, ln D_1m .. ln D_Nm)."""
-        S('const double lnT = log(T);')
+        S('// FP32 throughout; logarithms and reciprocals on the hardware approximations')
+        S('// (v_log_f32, v_rcp_f32: ~1 ulp), which the training features share')
         S(f'float raw[{N + 2}];')
         S('raw[0] = (float)((T - 1400.0) * (1.0 / 1100.0));')
-        S(f'raw[1] = (float)((log(p) - {lit(math.log(101325.0))}) * (1.0 / 1.3));')
+        S(f'const float lnp = kx_lnf((float)p);')
+        S(f'raw[1] = (lnp - {float(math.log(101325.0))!r}f) * {1.0 / 1.3!r}f;')
         for k in range(N):
             S(f'raw[{k + 2}] = (float)state[id + offsetT + {k} * offset];')
         S('// baselines (FP32): mole fractions, Wilke viscosity via rank-%d factors, conductivity'
           ' mixing rule, binary diffusion in the bath gas' % RANK)
-        S('const float lT = (float)lnT, lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
+        S('const float lT = kx_lnf((float)T), lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
         S('const float sqT = sqrtf((float)T);')
         for k in range(N):
             S(f'const float y{k} = fmaxf(0.0f, raw[{k + 2}]) * {float(rcpW[k])!r}f;')
-        S('const float rsum = 1.0f / (' + ' + '.join(f'y{k}' for k in range(N)) + ');')
+        S('const float ysum = ' + ' + '.join(f'y{k}' for k in range(N)) + ';')
+        S('const float rsum = __builtin_amdgcn_rcpf(ysum);')
         for k in range(N):
             S(f'const float x{k} = y{k} * rsum;')
         for k in range(N):
             S(f'const float e{k} = {fpoly(t["visc"][k])};')
-            S(f'const float ie{k} = 1.0f / e{k};')
+            S(f'const float ie{k} = __builtin_amdgcn_rcpf(e{k});')
         for m in range(RANK):
             for a, (U, V) in enumerate(fac):
                 if a == 0:
@@ -129,17 +132,17 @@ def emit(t, info):
             s2 = ' + '.join(f'{float(fac[1][0][k, m])!r}f * v1_{m}' for m in range(RANK))
             s3 = ' + '.join(f'{float(fac[2][0][k, m])!r}f * v2_{m}' for m in range(RANK))
             S(f'{{ const float Sk = ({s1}) + 2.0f * e{k} * ({s2}) + e{k} * e{k} * ({s3});')
-            S(f'  mu += x{k} * e{k} * e{k} / Sk; }}')
+            S(f'  mu += x{k} * e{k} * e{k} * __builtin_amdgcn_rcpf(Sk); }}')
         S('float l1 = 0.0f, l2 = 0.0f;')
         for k in range(N):
-            S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} / l; }}')
+            S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} * __builtin_amdgcn_rcpf(l); }}')
         S(f'float bl[{N + 2}];')
-        S('bl[0] = logf(sqT * mu);')
-        S('bl[1] = logf(sqT * 0.5f * (l1 + 1.0f / l2));')
-        S('const float lT15p = 1.5f * lT - (float)log(p);')
+        S('bl[0] = kx_lnf(sqT * mu);')
+        S('bl[1] = kx_lnf(sqT * 0.5f * (l1 + __builtin_amdgcn_rcpf(l2)));')
+        S('const float lT15p = 1.5f * lT - lnp;')
         for k in range(N):
             d = t['diff'][k, pr[k]]
-            expr = f'-logf({fpoly(d)})' if rcp_diff else f'logf({fpoly(d)})'
+            expr = f'-kx_lnf({fpoly(d)})' if rcp_diff else f'kx_lnf({fpoly(d)})'
             S(f'bl[{k + 2}] = lT15p + {expr};')
 
     def hull_and_inputs(store):
@@ -158,6 +161,7 @@ def emit(t, info):
             S(f'rcpMbar += fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
 
     S('#include "kx_mlp_wave.h"', 0)
+    S('__device__ __forceinline__ float kx_lnf(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }', 0)
     # ---------------- features (training and the composed inference path)
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
     S('kx_sur_features(const int n_states, const long long offsetT, const long long offset,', 0)
@@ -239,7 +243,8 @@ def emit(t, info):
     S('             const uint16_t* __restrict__ wfrag, const float* __restrict__ params,', 0)
     S('             int* __restrict__ flags, double* __restrict__ viscosity,', 0)
     S('             double* __restrict__ conductivity, double* __restrict__ rhoD,', 0)
-    S('             double* __restrict__ Dkm, const double Tref)', 0)
+    S('             double* __restrict__ Dkm, const double Tref,', 0)
+    S('             int* __restrict__ fb_queue, int* __restrict__ fb_count)', 0)
     S('{', 0)
     S(E.POOL_MARK)
     S(f'__shared__ __attribute__((aligned(16))) uint16_t xs[4][64 * {XS}];')
@@ -260,10 +265,11 @@ def emit(t, info):
     S(f'for (int i = 0; i < {IN}; i += 2)')
     S('  *(unsigned*)(xr + i) = kxmlp::pack2(xn[i], xn[i + 1]);')
     S(f'for (int i = 0; i < {N + 2}; ++i) br[i] = bl[i] + ostat[i];')
-    rcp_mbar()
-    S(f'rs[wv][lane] = p * {lit(1.0 / float(t["R"]))} / (T * rcpMbar);')
+    S(f'rs[wv][lane] = p * {lit(1.0 / float(t["R"]))} / (T * (double)ysum);')
     S('fs[wv][lane] = out;')
     S('flags[id] = out;')
+    S('// flagged states are queued for the plain kernel (vector atomic on global memory)')
+    S('if (out && fb_queue) fb_queue[atomicAdd(fb_count, 1)] = (int)id;')
     S('} else {')
     S(f'for (int i = 0; i < {IN}; i += 2) *(unsigned*)(xr + i) = 0u;')
     S('fs[wv][lane] = 1;')
@@ -294,5 +300,59 @@ def emit(t, info):
     S('    }')
     S('  }')
     S('}')
+    S('}', 0)
+
+    # ---------------- audit / fallback plumbing (queue of state ids -> compact plain run)
+    S('// counters: [0] states queued as out of hull, [1] failed audits, [2] valid queue entries')
+    S('// of the current compact pass')
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_queue_init(const int n_audit, const long long a0, const long long stride,', 0)
+    S('                  int* __restrict__ queue, int* __restrict__ counters)', 0)
+    S('{', 0)
+    S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (i < 4) counters[i] = 0;')
+    S('if (i < n_audit) queue[i] = (int)(a0 + i * stride);')
+    S('}', 0)
+    S('// Compact SoA copy (offsetT = offset = slots) of the queued states: entries [0, n_audit)')
+    S('// are audit samples, then min(counters[0] - skip, cap) out-of-hull states.')
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_gather(const int slots, const int n_audit, const int cap, const int skip,', 0)
+    S('              const int* __restrict__ queue, int* __restrict__ counters,', 0)
+    S('              const long long offsetT, const long long offset,', 0)
+    S('              const double* __restrict__ p_state, const double* __restrict__ state,', 0)
+    S('              double* __restrict__ cstate, double* __restrict__ cp)', 0)
+    S('{', 0)
+    S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (i >= slots) return;')
+    S('const int valid = n_audit + min(max(counters[0] - skip, 0), cap);')
+    S('if (i == 0) counters[2] = valid;')
+    S('if (i >= valid) return;')
+    S('const long long sid = queue[i];')
+    S('cstate[i] = state[sid];')
+    S(f'for (int k = 0; k < {N}; ++k) cstate[i + slots + k * (long long)slots] = state[sid + offsetT + k * offset];')
+    S('cp[i] = p_state[sid];')
+    S('}', 0)
+    S('// Plain results of the compact pass back into the outputs; audit entries (state not')
+    S('// flagged) first compare mu and lambda with the surrogate values they replace.')
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S('kx_sur_merge(const int slots, const int n_audit, const int* __restrict__ queue,', 0)
+    S('             int* __restrict__ counters, const int* __restrict__ flags, const long long offset,', 0)
+    S('             const double tol, const double* __restrict__ cvisc, const double* __restrict__ ccond,', 0)
+    S('             const double* __restrict__ crhoD, const double* __restrict__ cDkm,', 0)
+    S('             double* __restrict__ viscosity, double* __restrict__ conductivity,', 0)
+    S('             double* __restrict__ rhoD, double* __restrict__ Dkm)', 0)
+    S('{', 0)
+    S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (i >= slots || i >= counters[2]) return;')
+    S('const long long sid = queue[i];')
+    S('if (i < n_audit && flags[sid] == 0) {')
+    S('  const double ev = fabs(viscosity[sid] / cvisc[i] - 1.0);')
+    S('  const double ec = fabs(conductivity[sid] / ccond[i] - 1.0);')
+    S('  if (!(ev <= tol) || !(ec <= tol)) atomicAdd(counters + 1, 1);')
+    S('}')
+    S('viscosity[sid] = cvisc[i];')
+    S('conductivity[sid] = ccond[i];')
+    S(f'for (int k = 0; k < {N}; ++k) rhoD[k * offset + sid] = crhoD[k * (long long)slots + i];')
+    S(f'if (Dkm) for (int k = 0; k < {N}; ++k) Dkm[k * offset + sid] = cDkm[k * (long long)slots + i];')
     S('}', 0)
     return S.text()

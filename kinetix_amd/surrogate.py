@@ -7,13 +7,14 @@ kinetix_amd/gen/emit_surrogate.py. All arithmetic runs in this library's HIP ker
 inference, outputs, and the plain FP64 transport kernel for labels, audits and fallback.
 torch provides device memory, streams, random permutations and index gathers only.
 
-Inference safeguards (the fallback route is the plain kernel, kx_mixture_avg_transport_pstate):
+Inference safeguards (the fallback route is the plain kernel, kx_mixture_avg_transport_pstate),
+all inside one C-ABI call, kx_sur_transport:
   * hull check: any input outside the box spanned by the training inputs (small margin)
     -> plain kernel;
   * audit: every `audit_stride`-th state (1 % by default) is also computed by the plain
-    kernel; an audited state whose mu or lambda differs by more than `audit_tol` takes the
-    plain result; if more than `max_audit_fail` of the audited states fail, the whole call
-    falls back to the plain kernel.
+    kernel and keeps the plain result; one whose mu or lambda differs by more than
+    `audit_tol` counts as a failure; if more than `max_audit_fail` of the audited states
+    fail, the whole call falls back to the plain kernel.
 """
 from __future__ import annotations
 
@@ -32,13 +33,23 @@ IN = 64
 VALID = 55
 
 
+class SurStats(C.Structure):
+    _fields_ = [('n_states', C.c_int), ('out_of_hull', C.c_int), ('audited', C.c_int),
+                ('audit_failed', C.c_int), ('whole_call_fallback', C.c_int), ('fallback', C.c_int)]
+
+
 def _bind():
     L = lib()
     vp, ci, cf = C.c_void_p, C.c_int, C.c_float
     L.kx_sur_features.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_sur_targets.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_sur_outputs.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
-    L.kx_sur_infer.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_sur_infer.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.kx_sur_workspace_bytes.argtypes = [ci, ci, ci, ci]
+    L.kx_sur_workspace_bytes.restype = C.c_size_t
+    L.kx_sur_transport.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, C.c_double,
+                                   C.c_double, vp, vp, vp, vp, vp, C.c_size_t,
+                                   C.POINTER(SurStats), vp]
     L.kx_mlp_forward.argtypes = [vp, ci, vp, vp, vp]
     L.kx_mlp_backward.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
@@ -219,50 +230,21 @@ class Surrogate:
     # ---------------------------------------------------------------- inference
     def infer(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None,
               audit_stride=100, audit_tol=1e-3, max_audit_fail=0.01, audit_offset=0):
-        """Surrogate transport for n states (FP64 outputs in the plain kernel's layout).
-        Returns a dict with the fallback statistics."""
+        """Audited surrogate transport for n states (FP64 outputs in the plain kernel's layout),
+        one call of kx_sur_transport. Returns the fallback statistics."""
         import torch
-        N, dev = self.N, self.dev
-        if n == 0:
-            return {'n': 0, 'fallback': 0, 'fallback_fraction': 0.0}
-        flags = self._flags(n)
-        _check(self.L.kx_sur_infer(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state), _p(self.norm),
-                                   _p(self.ostat), _p(self.wbf16), _p(self.params), _p(flags),
-                                   _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm), _s()))
-        out_of_hull = int(flags.sum())
-        # audit: every audit_stride-th state through the plain kernel
-        aidx = torch.arange(audit_offset % max(audit_stride, 1), n, max(audit_stride, 1), device=dev)
-        n_fail = 0
-        if audit_stride > 0 and aidx.numel():
-            g_state, g_p = self._gather(aidx, offsetT, offset, d_state, d_p)
-            m = aidx.numel()
-            v, c, r, d = self.plain(m, m, m, g_p, g_state, Dkm=d_Dkm is not None)
-            ev = (d_visc.index_select(0, aidx) / v - 1).abs()
-            ec = (d_cond.index_select(0, aidx) / c - 1).abs()
-            bad = ((ev > audit_tol) | (ec > audit_tol) | ~torch.isfinite(ev) | ~torch.isfinite(ec))
-            bad = bad & (flags.index_select(0, aidx) == 0)
-            n_fail = int(bad.sum())
-            # audited states always take the plain result
-            self._scatter(aidx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm)
-        audited = int(aidx.numel()) if audit_stride > 0 else 0
-        whole = audited > 0 and n_fail > max_audit_fail * audited
-        if whole:
-            v, c, r, d = self.plain(n, offsetT, offset, d_p, d_state, Dkm=d_Dkm is not None)
-            d_visc.copy_(v); d_cond.copy_(c); d_rhoD[:r.numel()].copy_(r)
-            if d_Dkm is not None:
-                d_Dkm[:d.numel()].copy_(d)
-            n_fb = n
-        else:
-            fidx = torch.nonzero(flags, as_tuple=False).flatten()
-            if fidx.numel():
-                g_state, g_p = self._gather(fidx, offsetT, offset, d_state, d_p)
-                m = fidx.numel()
-                v, c, r, d = self.plain(m, m, m, g_p, g_state, Dkm=d_Dkm is not None)
-                self._scatter(fidx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm)
-            n_fb = out_of_hull + n_fail
-        return {'n': n, 'out_of_hull': out_of_hull, 'audited': audited, 'audit_failed': n_fail,
-                'whole_call_fallback': bool(whole), 'fallback': n_fb,
-                'fallback_fraction': n_fb / n}
+        need = int(self.L.kx_sur_workspace_bytes(self.kx.h, n, audit_stride, d_Dkm is not None))
+        if getattr(self, '_work', None) is None or self._work.numel() < need:
+            self._work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.dev)
+        st = SurStats()
+        _check(self.L.kx_sur_transport(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state),
+                                       _p(self.norm), _p(self.ostat), _p(self.wbf16), _p(self.params),
+                                       audit_stride, audit_offset, audit_tol, max_audit_fail,
+                                       _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm), _p(self._work),
+                                       self._work.numel(), C.byref(st), _s()))
+        return {'n': st.n_states, 'out_of_hull': st.out_of_hull, 'audited': st.audited,
+                'audit_failed': st.audit_failed, 'whole_call_fallback': bool(st.whole_call_fallback),
+                'fallback': st.fallback, 'fallback_fraction': st.fallback / n if n else 0.0}
 
     def infer_composed(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None):
         """The same model as three kernels (features -> kx_mlp_infer_bf16 -> outputs) with no
@@ -275,33 +257,6 @@ class Surrogate:
                                      _p(resid), _p(self.ostat), _p(flags), _p(d_visc), _p(d_cond),
                                      _p(d_rhoD), _p(d_Dkm), _s()))
         return flags
-
-    def _flags(self, n):
-        import torch
-        if getattr(self, '_flag_buf', None) is None or self._flag_buf.numel() < n:
-            self._flag_buf = torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
-        return self._flag_buf[:n]
-
-    def _gather(self, idx, offsetT, offset, d_state, d_p):
-        import torch
-        N = self.N
-        m = idx.numel()
-        g = torch.empty((N + 1, m), dtype=torch.float64, device=self.dev)
-        g[0] = d_state.index_select(0, idx)
-        rows = offsetT + torch.arange(N, device=self.dev)[:, None] * offset + idx[None, :]
-        g[1:] = d_state[rows]
-        return g.reshape(-1), d_p.index_select(0, idx)
-
-    def _scatter(self, idx, offset, m, v, c, r, d, d_visc, d_cond, d_rhoD, d_Dkm):
-        import torch
-        N = self.N
-        d_visc.index_copy_(0, idx, v)
-        d_cond.index_copy_(0, idx, c)
-        rows = (torch.arange(N, device=self.dev)[:, None] * offset + idx[None, :]).reshape(-1)
-        d_rhoD.index_copy_(0, rows, r.view(N, m).reshape(-1))
-        if d_Dkm is not None:
-            d_Dkm.index_copy_(0, rows, d.view(N, m).reshape(-1))
-
 
 def model_path(mech='gri30'):
     from .api import MECH_ROOT

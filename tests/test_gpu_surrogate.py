@@ -210,3 +210,26 @@ def test_fused_kernel_matches_composed_path(trained, gri, n):
         x_, y_ = x.view(-1, n)[:, ok], y.view(-1, n)[:, ok]
         assert torch.all(torch.isfinite(x_))
         assert float(((x_ - y_).abs() / y_.abs()).max()) <= 1e-5
+
+
+def test_out_of_hull_beyond_one_compact_pass(trained, gri):
+    """More out-of-hull states than one compact pass holds (4096 here): the remainder goes
+    through further passes; every state must equal the plain kernel's result."""
+    info, t = gri
+    kx, sur, _ = trained
+    n = 5000
+    s, p = states.sample_transport_set(n, info['species'], t['W'], seed=12)
+    s[0, :4700] = 3500.0
+    res, got, ref = _run(sur, kx, s, p)
+    assert res['out_of_hull'] == 4700 and not res['whole_call_fallback']
+    flagged = np.zeros(n, dtype=bool)
+    flagged[:4700] = True
+    for a, b in zip(got, ref):
+        a_, b_ = a.view(-1, n)[:, flagged], b.view(-1, n)[:, flagged]
+        assert torch.equal(a_, b_)
+    # audited states that were in hull also carry the plain result
+    aud = np.zeros(n, dtype=bool)
+    aud[::100] = True
+    aud &= ~flagged
+    for a, b in zip(got, ref):
+        assert torch.equal(a.view(-1, n)[:, aud], b.view(-1, n)[:, aud])
