@@ -90,6 +90,10 @@ DEFAULT_OPTS = {'sync': 'wave', 'nu_in_lds': True, 'kstep_outer': True, 'wform':
 def emit(t, info, opts=None):
     o = dict(DEFAULT_OPTS)
     o.update(opts or {})
+    if not o['nu_in_lds']:
+        # MFMA B operand read straight from global memory: this form produced wrong results
+        # on gfx950 (the LDS form, same arithmetic, is correct); disabled until explained
+        raise NotImplementedError('nu_in_lds=False is disabled (known-wrong results)')
     N = int(t['n_species'])
     p = plan(t)
     em = E._RateEmitter(t, info)

@@ -1,0 +1,61 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares; host-only
+calls and argument checks behave without a GPU (no kernel is launched here)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+from kinetix_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECL = re.compile(r'^\s*(?:const\s+)?(?:int|size_t|double|void|char)\s*\*?\s*(kx_\w+)\s*\(', re.M)
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, 'include', '*.h')):
+        names |= set(DECL.findall(open(h).read()))
+    return sorted(names)
+
+
+@pytest.fixture(scope='module')
+def L():
+    if not os.path.exists(api.LIB_PATH):
+        pytest.skip('libkinetix_mi.so not built (python -m kinetix_amd.build)')
+    return api.lib()
+
+
+def test_headers_declare_the_boundary():
+    names = declared()
+    for must in ('kx_init', 'kx_build', 'kx_finalize', 'kx_production_rates',
+                 'kx_production_rates_mixed', 'kx_mixture_avg_transport',
+                 'kx_mixture_avg_transport_pstate', 'kx_thermodynamic_props', 'kx_n_species',
+                 'kx_species_name', 'kx_sur_transport', 'kx_sur_workspace_bytes',
+                 'kx_mlp_forward', 'kx_mlp_backward', 'kx_mlp_adam', 'kx_mlp_infer_bf16'):
+        assert must in names, must
+
+
+def test_every_declared_symbol_is_exported(L):
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_only_queries(L):
+    assert L.kx_mlp_n_params() == 164672
+    assert L.kx_mlp_n_weights_bf16() == 163840
+    L.kx_sur_workspace_bytes.restype = C.c_size_t
+    L.kx_sur_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    assert L.kx_sur_workspace_bytes(987654, 1000, 100, 0) == 0         # unknown handle
+
+
+def test_unknown_handle_is_an_error_not_a_launch(L):
+    rc = L.kx_production_rates(987654, 16, 16, 16, 1.0, None, None, None)
+    assert rc == -2                                                    # KX_ERR_NOT_INITIALIZED
+    assert b'handle' in L.kx_last_error()
+    assert L.kx_n_species(987654) == -2
+    rc = L.kx_mlp_forward(None, 64, None, None, None)
+    assert rc == -1                                                    # KX_ERR_INVALID_ARG
+    rc = L.kx_mlp_forward(C.c_void_p(16), 65, C.c_void_p(16), C.c_void_p(16), None)
+    assert rc == -1                                                    # batch not a multiple of 64

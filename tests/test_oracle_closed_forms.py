@@ -1,0 +1,150 @@
+"""Pins the oracle where the reference's GRI fixtures cannot: single-reaction mechanisms
+against the textbook rate laws, and mixing rules against their one- and two-species limits.
+
+Textbook forms used (written from the publications, not from the oracle's code):
+  Arrhenius          k = A T^b exp(-Ea / (R T))
+  third body         k [M],  [M] = sum_k eff_k C_k
+  Lindemann          k = k_inf Pr / (1 + Pr),  Pr = k_0 [M] / k_inf
+  Troe (Gilbert, Luther and Troe, Ber. Bunsenges. Phys. Chem. 87 (1983) 169)
+                     F_cent = (1-a) e^{-T/T3} + a e^{-T/T1} + e^{-T2/T}
+                     log F = log F_cent / (1 + ((log Pr + c) / (n - 0.14 (log Pr + c)))^2)
+                     c = -0.4 - 0.67 log F_cent,  n = 0.75 - 1.27 log F_cent
+  SRI (Stewart, Larson and Golden, Combust. Flame 75 (1989) 25)
+                     F = d (a e^{-b/T} + e^{-T/c})^X T^e,  X = 1 / (1 + (log Pr)^2)
+Wilke viscosity with X_k = 1 reduces to mu_k; the conductivity rule to lambda_k; the
+mixture-averaged diffusion coefficient of a binary mixture to D_12 W_2 / Mbar.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from kinetix_amd.gen import mechanism, tables
+from oracle import pyoracle as po
+
+R_CAL = 8.31446261815324 / 4.184          # cal/mol/K (the CODATA R of the generator)
+NASA = {'model': 'NASA7', 'temperature-ranges': [200.0, 1000.0, 3500.0],
+        'data': [[3.5, 0.0, 0.0, 0.0, 0.0, -1000.0, 3.0], [3.5, 0.0, 0.0, 0.0, 0.0, -1000.0, 3.0]]}
+KINF = {'A': 3.0e14, 'b': 0.2, 'Ea': 90000.0}
+K0 = {'A': 2.0e18, 'b': -1.1, 'Ea': 80000.0}
+TROE = {'A': 0.6, 'T3': 120.0, 'T1': 1500.0, 'T2': 5000.0}
+SRI = {'A': 0.5, 'B': 200.0, 'C': 1000.0, 'D': 1.2, 'E': 0.1}
+EFF = {'O2': 2.5, 'N2': 0.7}
+
+
+def _model():
+    sp = [{'name': n, 'composition': c, 'thermo': NASA}
+          for n, c in (('H2', {'H': 2}), ('H', {'H': 1}), ('O2', {'O': 2}), ('N2', {'N': 2}))]
+    rx = [
+        {'equation': 'H2 => 2 H', 'rate-constant': {'A': 1.0e14, 'b': 0.5, 'Ea': 50000.0}},
+        {'equation': 'H2 + M => 2 H + M', 'type': 'three-body',
+         'rate-constant': {'A': 1.0e18, 'b': -1.0, 'Ea': 100000.0}, 'efficiencies': EFF},
+        {'equation': 'H2 (+M) => 2 H (+M)', 'type': 'falloff', 'high-P-rate-constant': KINF,
+         'low-P-rate-constant': K0, 'efficiencies': EFF},
+        {'equation': 'H2 (+M) => 2 H (+M)', 'type': 'falloff', 'high-P-rate-constant': KINF,
+         'low-P-rate-constant': K0, 'Troe': TROE, 'efficiencies': EFF, 'duplicate': True},
+        {'equation': 'H2 (+M) => 2 H (+M)', 'type': 'falloff', 'high-P-rate-constant': KINF,
+         'low-P-rate-constant': K0, 'SRI': SRI, 'efficiencies': EFF, 'duplicate': True},
+    ]
+    return {'units': {'length': 'cm', 'time': 's', 'quantity': 'mol', 'activation-energy': 'cal/mol'},
+            'species': sp, 'reactions': rx}
+
+
+@pytest.fixture(scope='module')
+def mini():
+    mech = mechanism.from_model(_model(), 'mini')
+    t = tables.build(mech)
+    if not os.path.exists(os.path.join(os.path.dirname(po.__file__), 'lib', 'libkxoracle.so')):
+        po.build_libs(with_ref=False)
+    return mech, t, po.Oracle(t)
+
+
+def _conc(t, T, Y, p):
+    W = np.asarray(t['W'])
+    rho = p / (float(t['R']) * T) / np.sum(np.asarray(Y) / W)
+    return rho * np.asarray(Y) / W
+
+
+def _arr(c, T, order):
+    return c['A'] * 1e-6 ** order * T ** c['b'] * math.exp(-c['Ea'] / (R_CAL * T))
+
+
+@pytest.mark.parametrize('T,p', [(800.0, 101325.0), (1500.0, 5e3), (2200.0, 4e6), (1200.0, 101325.0 * 40)])
+def test_single_reaction_rate_laws(mini, T, p):
+    mech, t, orc = mini
+    # species order: H2 H O2 N2 (all take part, none inert)
+    Y = np.array([0.3, 0.01, 0.29, 0.4])
+    s = np.zeros((5, 1))
+    s[0, 0] = T
+    s[1:, 0] = Y
+    q = orc.reaction_rates(s, p)[:, 0]
+    C = _conc(t, T, Y, p)
+    M_all = C.sum()
+    M_eff = C.sum() + (EFF['O2'] - 1) * C[2] + (EFF['N2'] - 1) * C[3]
+    k1 = _arr({'A': 1.0e14, 'b': 0.5, 'Ea': 50000.0}, T, 0)
+    k2 = _arr({'A': 1.0e18, 'b': -1.0, 'Ea': 100000.0}, T, 1)
+    kinf, k0 = _arr(KINF, T, 0), _arr(K0, T, 1)
+    Pr = k0 * M_eff / kinf
+    lind = kinf * Pr / (1 + Pr)
+    Fc = (1 - TROE['A']) * math.exp(-T / TROE['T3']) + TROE['A'] * math.exp(-T / TROE['T1']) \
+        + math.exp(-TROE['T2'] / T)
+    c = -0.4 - 0.67 * math.log10(Fc)
+    n = 0.75 - 1.27 * math.log10(Fc)
+    x = math.log10(Pr) + c
+    F_troe = 10 ** (math.log10(Fc) / (1 + (x / (n - 0.14 * x)) ** 2))
+    X = 1 / (1 + math.log10(Pr) ** 2)
+    F_sri = SRI['D'] * (SRI['A'] * math.exp(-SRI['B'] / T) + math.exp(-T / SRI['C'])) ** X * T ** SRI['E']
+    expect = np.array([k1, k2 * M_eff, lind, lind * F_troe, lind * F_sri]) * C[0]
+    np.testing.assert_allclose(q, expect, rtol=1e-12, atol=0)
+    assert M_eff != M_all                           # the efficiencies matter in this state
+
+
+def test_falloff_pressure_limits(mini):
+    """Lindemann form: Pr -> inf gives k_inf, Pr -> 0 gives k_0 [M]."""
+    mech, t, orc = mini
+    T = 1400.0
+    Y = np.array([0.3, 0.01, 0.29, 0.4])
+    s = np.zeros((5, 1)); s[0, 0] = T; s[1:, 0] = Y
+    kinf, k0 = _arr(KINF, T, 0), _arr(K0, T, 1)
+    hi = orc.reaction_rates(s, 1e18)[:, 0] / _conc(t, T, Y, 1e18)[0]
+    lo_p = 1e-6
+    Clo = _conc(t, T, Y, lo_p)
+    M_eff = Clo.sum() + (EFF['O2'] - 1) * Clo[2] + (EFF['N2'] - 1) * Clo[3]
+    lo = orc.reaction_rates(s, lo_p)[:, 0] / Clo[0]
+    assert hi[2] == pytest.approx(kinf, rel=1e-6)
+    assert lo[2] == pytest.approx(k0 * M_eff, rel=1e-9)
+
+
+def test_pure_species_and_binary_transport_limits(oracle, gri):
+    """Pure species: Wilke -> mu_k, conductivity rule -> lambda_k (the fitted pure-species
+    values at T). Binary H2/N2: D_1m = D_12 W_2 / Mbar."""
+    info, t = gri
+    sp = info['species']
+    N = len(sp)
+    T = np.array([400.0, 1000.0, 1800.0, 2900.0])
+    lnT = np.log(T)
+    P = np.stack([lnT ** i for i in range(5)])
+    for name in ('H2', 'N2', 'H2O', 'CH4'):
+        k = sp.index(name)
+        s = np.zeros((N + 1, len(T)))
+        s[0] = T
+        s[1 + k] = 1.0
+        cond, visc, rhoD, _ = oracle.transport(s, 1.0)
+        mu_k = np.sqrt(T) * (np.asarray(t['visc'])[k] @ P) ** 2
+        lam_k = np.sqrt(T) * (np.asarray(t['cond'])[k] @ P)
+        np.testing.assert_allclose(visc, mu_k, rtol=1e-12)
+        np.testing.assert_allclose(cond, lam_k, rtol=1e-12)
+    i1, i2 = sp.index('H2'), sp.index('N2')
+    W = np.asarray(t['W'])
+    Y1 = 0.2
+    s = np.zeros((N + 1, len(T)))
+    s[0] = T
+    s[1 + i1], s[1 + i2] = Y1, 1 - Y1
+    p = 101325.0
+    _, _, _, Dkm = oracle.transport(s, 1.0, p_state=np.full(len(T), p))
+    Mbar = 1.0 / (Y1 / W[i1] + (1 - Y1) / W[i2])
+    a, b = max(i1, i2), min(i1, i2)
+    d = np.asarray(t['diff'])[a, b] @ P
+    D12 = (T ** 1.5 / (d * p)) if int(t.get('rcp_diff', 0)) else d * T ** 1.5 / p
+    np.testing.assert_allclose(Dkm[i1], D12 * W[i2] / Mbar, rtol=1e-12)
