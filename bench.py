@@ -28,6 +28,12 @@ PEAK_FP32 = 157.3     # TFLOP/s
 PEAK_HBM = 8000.0     # GB/s
 PEAK_BF16 = 2500.0    # TFLOP/s, dense BF16 MFMA
 MLP_FLOPS = 2 * (55 * 256 + 256 * 256 * 2 + 256 * 55)   # useful MACs x 2 per state
+
+
+def SUR_BYTES(N):
+    """HBM bytes per state of the fused surrogate kernel: T, Y_1..Y_N and p in; mu, lambda,
+    rho*D_k and the hull flag out (FP64 except the flag)."""
+    return 8 * (N + 1) + 8 + 8 * (N + 2) + 4
 P_ATM = 101325.0
 
 
@@ -64,10 +70,11 @@ def host_cores():
     return max(1, min(n, 16))
 
 
-def cpu_baseline(state, p, cores, target_s):
-    """Time the reference's own FP64 GRI routines (BK1 + BK2 per state) on this host."""
+def cpu_baseline(state, p, cores, target_s, mech='gri30'):
+    """Time the reference's own FP64 GRI routines (BK1 + BK2 per state) on this host; for
+    other mechanisms (no reference build exists) the oracle port with the same flags."""
     from oracle import pyoracle as po
-    kind = 'reference' if po.Reference.available('fast') else 'port'
+    kind = 'reference' if (mech == 'gri30' and po.Reference.available('fast')) else 'port'
     n_all = state.shape[1]
     if kind == 'reference':
         ref = po.Reference('fast')
@@ -77,7 +84,7 @@ def cpu_baseline(state, p, cores, target_s):
             ref.transport(s, 1.0, threads=cores)
     else:
         from kinetix_amd.gen import tables
-        _, t = tables.load(os.path.join(ROOT, 'kinetix_amd', 'mechs', 'gri30'))
+        _, t = tables.load(os.path.join(ROOT, 'kinetix_amd', 'mechs', mech))
         orc = po.Oracle(t, fast=True)
 
         def run(s):
@@ -115,11 +122,11 @@ def main():
     counts = cnt.load(mdir)
     kx = Kinetix(a.mech)
     sp, W = kx.speciesNames(), kx.molar_masses
-    kx.build(P_ATM, 1.0, states.air_mass_fractions(sp, W))
+    kx.build(P_ATM, 1.0, states.reference_mass_fractions(sp, W))
     N, n = kx.nSpecies(), a.states
 
     t0 = time.time()
-    st, _ = states.sample(n, sp, W, T_range=(1000.0, 2500.0), seed=states.BASE_SEED + 2)
+    st, _ = states.sample_any(n, sp, W, T_range=(1000.0, 2500.0), seed=states.BASE_SEED + 2)
     gen_s = time.time() - t0
     dev = torch.device('cuda')
     d_state = torch.from_numpy(st).to(dev).reshape(-1)
@@ -134,6 +141,11 @@ def main():
     bk2_name = 'kx_bk2_fp64' if a.bk2 == 'fp64' else 'kx_mlp_infer_bf16'
     d_p = torch.full((n,), P_ATM, dtype=torch.float64, device=dev)
     sur = None
+    if a.bk2 == 'surrogate' and N + 2 > 64:
+        print(f'bench.py: {a.mech} has {N} species; the surrogate covers at most 62, '
+              f'timing the plain BK2', file=sys.stderr)
+        a.bk2 = 'fp64'
+        bk2_name = 'kx_bk2_fp64'
     if a.bk2 == 'surrogate':
         from kinetix_amd.surrogate import Surrogate, model_path
         path = a.surrogate_model or model_path(a.mech)
@@ -193,8 +205,13 @@ def main():
     idx = np.random.default_rng(7).choice(n, min(a.err_sample, n), replace=False)
     ref, scale = orc.production_rates(st[:, idx], P_ATM, scale=True, threads=cores)
     variants = {}
+    from kinetix_amd.api import KinetixError
     for vname, (kname, mode) in BK1.items():
-        bk1(mode); torch.cuda.synchronize()
+        try:
+            bk1(mode); torch.cuda.synchronize()
+        except KinetixError as e:          # form not generated for this mechanism
+            variants[vname] = {'kernel': kname, 'absent': str(e)}
+            continue
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(5):
@@ -207,7 +224,8 @@ def main():
         variants[vname] = {'kernel': kname, 'ms': round(ms, 4), 'Mstates_s': round(n / ms / 1e3, 2),
                            'max_rel_l2_per_species_vs_fp64_oracle': max(l2v)}
     for v in variants.values():
-        v['speedup_vs_fp64'] = round(variants['fp64']['ms'] / v['ms'], 3)
+        if 'ms' in v:
+            v['speedup_vs_fp64'] = round(variants['fp64']['ms'] / v['ms'], 3)
     bk1(); torch.cuda.synchronize()               # leave the headline variant's output
 
     # BK2 forms on the same inputs; the plain FP64 kernel is the judge of the surrogate over
@@ -240,7 +258,7 @@ def main():
                    'audit_failed': st_['audit_failed'],
                    'whole_call_fallback': st_['whole_call_fallback'],
                    'states_compared': n}
-        bk2_variants['surrogate'] = dict({'kernel': 'kx_mlp_infer_bf16 (+features/outputs, 1% audit)',
+        bk2_variants['surrogate'] = dict({'kernel': 'kx_sur_transport (kx_sur_infer + 1% audit + fallback)',
                                           'ms': round(ms_s, 4)}, **sur_err)
     for v in bk2_variants.values():
         v['Mstates_s'] = round(n / v['ms'] / 1e3, 2)
@@ -266,15 +284,15 @@ def main():
            'bk2_rhoD_max_rel': rel(rhoD, r0), 'sample_states': int(len(idx))}
 
     def roof(name, ms):
-        if name == 'kx_mlp_infer_bf16':
+        if name == 'kx_sur_infer':
             fl = MLP_FLOPS * n / (ms * 1e-3) / 1e12
+            gbs = SUR_BYTES(N) * n / (ms * 1e-3) / 1e9
             return {'bound': 'mfma', 'pipe': 'BF16 MFMA (dense peak 2.5 PFLOP/s)',
                     'achieved': round(fl, 3), 'peak': PEAK_BF16, 'unit': 'TFLOP/s',
                     'frac': round(fl / PEAK_BF16, 4), 'traffic': None,
-                    'flops_per_state': MLP_FLOPS, 'avg_launch_ms': round(ms, 4),
-                    'states_per_launch': n,
-                    'note': 'avg_launch_ms is the whole surrogate call (features, MLP, outputs, '
-                            'audit); the MLP kernel alone is timed by rocprofv3'}
+                    'flops_per_state': MLP_FLOPS, 'bytes_per_state': SUR_BYTES(N),
+                    'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
+                    'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
         c = counts[name if name in counts else 'kx_bk1_fp64']
         fl = c['flops'] * n / (ms * 1e-3) / 1e12
         gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
@@ -283,12 +301,30 @@ def main():
                 'achieved': round(fl, 3), 'peak': peak, 'unit': 'TFLOP/s',
                 'frac': round(fl / peak, 4), 'traffic': None,
                 'flops_per_state': c['flops'], 'transcendentals_per_state': c['transcendentals'],
+                'bytes_per_state': c['bytes'],
                 'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
                 'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
 
     m1, m2 = float(t_bk1.mean()), float(t_bk2.mean())
-    kern = {bk1_name: roof(bk1_name, m1), bk2_name: roof(bk2_name, m2)}
-    dom = bk1_name if m1 >= m2 else bk2_name
+    kern = {bk1_name: roof(bk1_name, m1)}
+    if sur is not None:
+        # the fused surrogate kernel on its own, event-timed on the launch stream
+        from kinetix_amd.surrogate import _p, _s
+        fl_buf = torch.empty(n, dtype=torch.int32, device=dev)
+
+        def sur_kernel():
+            sur.L.kx_sur_infer(kx.h, n, n, n, _p(d_p), _p(d_state), _p(sur.norm), _p(sur.ostat),
+                               _p(sur.wbf16), _p(sur.params), _p(fl_buf), _p(d_visc), _p(d_cond),
+                               _p(d_rhoD), None, None, None, _s())
+        m2k = timed(sur_kernel)
+        kern['kx_sur_infer'] = roof('kx_sur_infer', m2k)
+        bk2_kernel_ms = m2k
+        bk2() ; torch.cuda.synchronize()              # restore the audited outputs
+    else:
+        kern[bk2_name] = roof(bk2_name, m2)
+        bk2_kernel_ms = m2
+    bk2_kernel = 'kx_sur_infer' if sur is not None else bk2_name
+    dom = bk1_name if m1 >= bk2_kernel_ms else bk2_kernel
     roofline = dict(kern[dom]); roofline['kernel'] = dom
     traffic_path = os.path.join(ROOT, 'profiles', 'traffic.json')
     if os.path.exists(traffic_path):
@@ -325,7 +361,7 @@ def main():
     if sur is not None:
         out['surrogate'] = sur_err
     if not a.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(st, P_ATM, cores, a.cpu_seconds)
+        out['cpu_baseline'] = cpu_baseline(st, P_ATM, cores, a.cpu_seconds, a.mech)
     print(json.dumps(out))
     return 0
 

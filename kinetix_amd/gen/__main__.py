@@ -21,6 +21,8 @@ def main(argv=None):
                     help='fit T^1.5/(p D_jk) instead of p D_jk/T^1.5 (no divides in BK2)')
     ap.add_argument('--no-transport', action='store_true', help='skip transport fits/kernels')
     ap.add_argument('--tables-only', action='store_true', help='write tables, no kernels')
+    ap.add_argument('--rolled', action='store_true',
+                    help='loop forms of the FP64/mixed kernels (default above 96 species)')
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -34,7 +36,7 @@ def main(argv=None):
     counts.write(args.output, tab)
     if not args.tables_only:
         from . import emit_hip
-        emit_hip.write(args.output, mech, tab)
+        emit_hip.write(args.output, mech, tab, {'rolled': True} if args.rolled else None)
     print(f'{mech.name}: {mech.n_species} species ({mech.n_active} active), '
           f'{mech.n_reactions} reactions -> {args.output} [{time.time() - t0:.1f}s]')
     return 0

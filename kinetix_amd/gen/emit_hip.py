@@ -406,8 +406,9 @@ class _RateEmitter:
             ratio = math.log2(t['A0'][r] / t['A'][r])
             db = float(t['beta0'][r]) - float(t['beta'][r])
             dT = float(t['Ta0'][r]) - float(t['Ta'][r])
-            S(f'  const float Pr = (float)({self._third_w(r, sidx)} * '
-              f'{pow2}({self._log2_rate(2.0 ** ratio, db, dT)}));')
+            # clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32
+            S(f'  const float Pr = (float)fmin({self._third_w(r, sidx)} * '
+              f'{pow2}({self._log2_rate(2.0 ** ratio, db, dT)}), 1e30);')
             S('  const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
             if kind == M.LINDEMANN:
                 S('  F *= (double)Pr1;')
@@ -775,6 +776,17 @@ def generate(tables, info, opts=None):
     o.update(opts or {})
     parts = [HEADER.format(name=info['name'], n_species=info['n_species'],
                            n_active=info['n_active_species'], n_reactions=info['n_reactions'])]
+    from . import emit_rolled
+    if o.get('rolled') or int(tables['n_species']) > emit_rolled.ROLL_N:
+        # large mechanism: loop forms (private per-lane species arrays, uniform table loads);
+        # no MFMA-accumulation variant and no surrogate (its width is fixed at 64)
+        parts.append(emit_rolled.emit_bk1(tables, info, 'fp64'))
+        parts.append(emit_rolled.emit_bk1(tables, info, 'mixed_valu'))
+        if 'cond' in tables:
+            parts.append(emit_rolled.emit_bk2(tables, info, False))
+            parts.append(emit_rolled.emit_bk2(tables, info, True))
+        parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
+        return '\n\n'.join(parts) + '\n'
     parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info, o['bk1_sched_window'], o['bk1_wform'], lds=o['lds_pool']))
     if 'cond' in tables:
         parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False, lds=o['lds_pool']))

@@ -1,0 +1,354 @@
+"""Loop ("rolled") forms of the FP64 kernels for large mechanisms (gfx950 HIP).
+
+The straight-line kernels of emit_hip keep every species quantity in registers and expand
+every pair term; past ~100 species that code no longer fits the register file or the
+instruction cache (and takes hours to compile). The rolled forms keep the per-state
+species arrays in per-lane private memory (scratch, interleaved across lanes so every
+array access of a wavefront is one contiguous line), read the mechanism's coefficients as
+uniform (scalar-cache) loads from __device__ tables, and loop over species, pairs and
+reactions. Same arithmetic and summation order as the straight-line forms:
+
+  BK2: kinetix/core/mix_transport.py:261-471 and benchmark/okl/transportProps.okl:11-47
+       (symmetric diffusion pair loop, Wilke viscosity, conductivity mixing rule);
+  BK1: kinetix/core/reaction_rates.py:616-1333 and benchmark/okl/productionRates.okl:10-62
+       (rolled reaction loop; reaction descriptors in tables).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import emit_hip as E
+from . import mechanism as M
+
+ROLL_N = 96          # mechanisms with more species than this get the loop forms
+
+
+def _table(name, arr, ctype='double'):
+    a = np.asarray(arr).reshape(-1)
+    if ctype == 'double':
+        body = ', '.join(repr(float(v)) for v in a)
+    else:
+        body = ', '.join(str(int(v)) for v in a)
+    return f'__device__ const {ctype} {name}[{max(a.size, 1)}] = {{{body if a.size else "0"}}};'
+
+
+def _poly(tab, row):
+    return (f'({tab}[5 * ({row})] + {tab}[5 * ({row}) + 1] * lnT + {tab}[5 * ({row}) + 2] * lnT2 + '
+            f'{tab}[5 * ({row}) + 3] * lnT3 + {tab}[5 * ({row}) + 4] * lnT4)')
+
+
+def emit_bk2(t, info, pstate=False):
+    N = int(t['n_species'])
+    rcp_diff = bool(int(t.get('rcp_diff', 0)))
+    name = 'kx_bk2_fp64_pstate' if pstate else 'kx_bk2_fp64'
+    pre = f'{name}_r'
+    S = E._Src()
+    d = np.asarray(t['diff'])
+    pairs = np.array([d[k, j] for k in range(1, N) for j in range(k)])
+    S(_table(f'{pre}_rcpW', t['rcpW']), 0)
+    S(_table(f'{pre}_W', t['W']), 0)
+    S(_table(f'{pre}_visc', t['visc']), 0)
+    S(_table(f'{pre}_cond', t['cond']), 0)
+    S(_table(f'{pre}_diff', pairs), 0)
+    S(_table(f'{pre}_C1', t['C1']), 0)
+    S(_table(f'{pre}_C2', t['C2']), 0)
+    S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
+    if pstate:
+        S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
+        S('            const double* __restrict__ p_state,', 0)
+    else:
+        S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
+        S('            const double pressure,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
+    S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
+    S('            ' + ('double* __restrict__ Dkm, ' if pstate else '') + 'const double Tref,', 0)
+    S('            const int* __restrict__ n_dev)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
+    S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
+    S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
+    S('const double T = Tref * state[id];')
+    S('const double lnT = log(T);')
+    S('const double rcpT = 1.0 / T;')
+    S('const double sqrT = sqrt(T);')
+    S('const double lnT2 = lnT * lnT;')
+    S('const double lnT3 = lnT * lnT * lnT;')
+    S('const double lnT4 = lnT * lnT * lnT * lnT;')
+    S(f'double x[{N}], s[{N}], re[{N}];       // per-lane private arrays')
+    S('double rcpMbar = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  x[k] = fmax(0.0, state[id + offsetT + k * offset]) * {pre}_rcpW[k];')
+    S('  rcpMbar += x[k];')
+    S('}')
+    S('const double Mbar = 1.0 / rcpMbar;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{ x[k] *= Mbar; s[k] = 0.0; }}')
+    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj')
+    S('int pr = 0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 1; k < {N}; ++k) {{')
+    S('  const double xk = x[k];')
+    S('  double sk = s[k];')
+    S('  #pragma unroll 2')
+    S('  for (int j = 0; j < k; ++j, ++pr) {')
+    if rcp_diff:
+        S(f'    const double rD = {_poly(pre + "_diff", "pr")};')
+    else:
+        S(f'    const double rD = 1.0 / {_poly(pre + "_diff", "pr")};')
+    S('    sk += x[j] * rD;')
+    S('    s[j] += xk * rD;')
+    S('  }')
+    S('  s[k] = sk;')
+    S('}')
+    S('const double TsqrT = T * sqrT;')
+    S(f'const double rho = p * {repr(1.0 / float(t["R"]))} * rcpT * Mbar;')
+    S('const double pMbar = p * Mbar;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  const double D = TsqrT * (Mbar - {pre}_W[k] * x[k]) / (pMbar * s[k]);')
+    S('  rhoD[k * offset + id] = rho * D;')
+    if pstate:
+        S('  Dkm[k * offset + id] = D;')
+    S('}')
+    S('// ---- viscosity (Wilke)')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) re[k] = 1.0 / {_poly(pre + "_visc", "k")};')
+    S('double vis = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  const double e = {_poly(pre + "_visc", "k")};')
+    S('  double sv = 0.0;')
+    S('  #pragma unroll 4')
+    S(f'  for (int j = 0; j < {N}; ++j) {{')
+    S(f'    const double f = {pre}_C1[k * {N} + j] + {pre}_C2[k * {N} + j] * e * re[j];')
+    S('    sv += x[j] * f * f;')
+    S('  }')
+    S('  vis += x[k] * e * e / sv;')
+    S('}')
+    S('viscosity[id] = sqrT * vis;')
+    S('// ---- conductivity')
+    S('double sum1 = 0.0, sum2 = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  const double l = {_poly(pre + "_cond", "k")};')
+    S('  sum1 += x[k] * l;')
+    S('  sum2 += x[k] / l;')
+    S('}')
+    S('conductivity[id] = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+    S('}', 0)
+    return S.text()
+
+
+def _bk1_tables(t, pre):
+    """Reaction descriptors of the rolled BK1 (one row per reaction)."""
+    N, R = int(t['n_species']), int(t['n_reactions'])
+    nu_f, nu_r = np.asarray(t['nu_f']).reshape(R, N), np.asarray(t['nu_r']).reshape(R, N)
+    kind, rev = np.asarray(t['kind']), np.asarray(t['reversible'])
+    collider, has_eff = np.asarray(t['collider']), np.asarray(t['has_eff'])
+    eff = np.asarray(t['eff']).reshape(R, N)
+    L2E = 1.4426950408889634
+    # unique efficiency sets (sparse: species, eff - 1)
+    sets, set_of = [], {}
+    for r in range(R):
+        if has_eff[r]:
+            key = tuple(eff[r])
+            if key not in set_of:
+                set_of[key] = len(sets)
+                sets.append(key)
+    set_off, set_sp, set_e = [0], [], []
+    for key in sets:
+        for k, e in enumerate(key):
+            if e != 1.0:
+                set_sp.append(k)
+                set_e.append(e - 1.0)
+        set_off.append(len(set_sp))
+    # per reaction: expanded reactant/product lists (<= 4 each), net stoichiometry (<= 8)
+    MX, MS = 4, 8
+    f_sp = np.full((R, MX), -1); r_sp = np.full((R, MX), -1)
+    n_sp = np.full((R, MS), -1); n_nu = np.zeros((R, MS))
+    meta = np.zeros((R, 8), dtype=np.int64)     # kind, rev, nf, nr, nnet, dnu, mkind, midx
+    dbl = np.zeros((R, 13))                     # log2A, beta, TaL2E, l2ratio, dbeta, dTaL2E, troe(4), (sri a b rc d e in 6..10)
+    for r in range(R):
+        fl = [k for k in range(N) for _ in range(int(nu_f[r, k]))]
+        rl = [k for k in range(N) for _ in range(int(nu_r[r, k]))]
+        if len(fl) > MX or len(rl) > MX:
+            raise ValueError(f'reaction {r}: more than {MX} reactant/product molecules')
+        f_sp[r, :len(fl)] = fl
+        r_sp[r, :len(rl)] = rl
+        net = [(k, int(nu_r[r, k] - nu_f[r, k])) for k in range(N) if nu_r[r, k] != nu_f[r, k]]
+        if len(net) > MS:
+            raise ValueError(f'reaction {r}: more than {MS} species')
+        for i, (k, v) in enumerate(net):
+            n_sp[r, i], n_nu[r, i] = k, v
+        if has_eff[r]:
+            mk, mi = 3, set_of[tuple(eff[r])]
+        elif collider[r] >= 0:
+            mk, mi = 2, int(collider[r])
+        else:
+            mk, mi = 1, 0
+        meta[r] = (kind[r], rev[r], len(fl), len(rl), len(net), int(sum(v for _, v in net)), mk, mi)
+        A, b, Ta = float(t['A'][r]), float(t['beta'][r]), float(t['Ta'][r])
+        row = [math.log2(A), b, Ta * L2E] + [0.0] * 10
+        if kind[r] in (M.LINDEMANN, M.TROE, M.SRI):
+            row[3] = math.log2(float(t['A0'][r]) / A)
+            row[4] = float(t['beta0'][r]) - b
+            row[5] = (float(t['Ta0'][r]) - Ta) * L2E
+        if kind[r] == M.TROE:
+            row[6:10] = [float(v) for v in np.asarray(t['troe']).reshape(R, 4)[r]]
+        if kind[r] == M.SRI:
+            row[6:11] = [float(v) for v in np.asarray(t['sri']).reshape(R, 5)[r]]
+        if kind[r] == M.PLOG:
+            raise NotImplementedError('PLOG kernels are not emitted yet')
+        dbl[r] = row
+    out = [_table(f'{pre}_fsp', f_sp, 'int'), _table(f'{pre}_rsp', r_sp, 'int'),
+           _table(f'{pre}_nsp', n_sp, 'int'), _table(f'{pre}_nnu', n_nu),
+           _table(f'{pre}_meta', meta, 'int'), _table(f'{pre}_dbl', dbl),
+           _table(f'{pre}_setoff', np.array(set_off), 'int'), _table(f'{pre}_setsp', np.array(set_sp), 'int'),
+           _table(f'{pre}_sete', np.array(set_e)),
+           _table(f'{pre}_rcpW', t['rcpW']), _table(f'{pre}_W', t['W']),
+           _table(f'{pre}_tmid', t['T_mid']), _table(f'{pre}_glo', t['g_lo']), _table(f'{pre}_ghi', t['g_hi']),
+           _table(f'{pre}_hlo', t['h_lo']), _table(f'{pre}_hhi', t['h_hi'])]
+    return out, len(sets), MX, MS
+
+
+def emit_bk1(t, info, variant='fp64'):
+    """Rolled BK1. variant 'fp64': FP64 exponentials and falloff; 'mixed_valu': Arrhenius
+    through kx_pow2 (FP64 exponent, FP32 exp2 of its fraction), falloff in FP32, Gibbs
+    terms, net rates and the wdot sums in FP64 -- the straight-line forms' split."""
+    N, R = int(t['n_species']), int(t['n_reactions'])
+    name = f'kx_bk1_{variant}'
+    pre = f'{name}_r'
+    mixed = variant != 'fp64'
+    tabs, nsets, MX, MS = _bk1_tables(t, pre)
+    S = E._Src()
+    for line in tabs:
+        S(line, 0)
+    S(f'__device__ __forceinline__ double {pre}_pow2(double y)', 0)
+    S('{', 0)
+    if mixed:
+        S('const double n = __builtin_rint(y);')
+        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
+        S('return __builtin_ldexp((double)m, (int)n);')
+    else:
+        S('return exp2(y);')
+    S('}', 0)
+    S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
+    S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('            const double pressure_R, const double pressure_,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('(void)pressure_;')
+    S('const double T = Tref * state[id];')
+    S('const double rcpT = 1.0 / T;')
+    S('const double lnT = log(T);')
+    S('const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;')
+    S('const double log2T = lnT * 1.4426950408889634;')
+    S('const float Tf = (float)T, rcpTf = (float)rcpT;')
+    S('(void)Tf; (void)rcpTf;')
+    S(f'double w[{N}], gl[{N}], wd[{N}], Ms[{max(nsets, 1)}];   // per-lane private arrays')
+    S('double rcpMbar = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{ const double c = fmax(0.0, state[id + offsetT + k * offset]) * {pre}_rcpW[k]; w[k] = c; rcpMbar += c; }}')
+    S('const double rho = pressure_R * rcpT / rcpMbar;')
+    S('double Cm = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{ w[k] *= rho; Cm += w[k]; wd[k] = 0.0; }}')
+    S('// third-body concentrations of the efficiency sets')
+    S('#pragma unroll 1')
+    S(f'for (int s = 0; s < {nsets}; ++s) {{')
+    S('  double m = Cm;')
+    S(f'  for (int i = {pre}_setoff[s]; i < {pre}_setoff[s + 1]; ++i) m += {pre}_sete[i] * w[{pre}_setsp[i]];')
+    S('  Ms[s] = m;')
+    S('}')
+    S('// Gibbs terms: gl = g log2(e), w = C exp(g) (FP64); colliders read C before this')
+    S(f'double Ccol[{N}];')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  const double* a = (T > {pre}_tmid[k] ? {pre}_ghi : {pre}_glo) + 7 * k;')
+    S('  const double g = a[0] * (1.0 - lnT) + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4 + a[5] * rcpT - a[6];')
+    S('  gl[k] = g * 1.4426950408889634;')
+    S('  Ccol[k] = w[k];')
+    S('  w[k] *= exp(g);')
+    S('}')
+    S(f'const double C0 = {repr(float(t["one_atm_R"]))} * rcpT;')
+    S(f'const double rcpC0 = {repr(float(t["R_one_atm"]))} * T;')
+    S('#pragma unroll 1')
+    S(f'for (int r = 0; r < {R}; ++r) {{')
+    S(f'  const int* mt = {pre}_meta + 8 * r;')
+    S(f'  const double* dp = {pre}_dbl + 13 * r;')
+    S('  const int kind = mt[0], nf = mt[2], nr = mt[3];')
+    S('  double ex = dp[0] + dp[1] * log2T - dp[2] * rcpT;')
+    S('  double fwd = 1.0, rev = 1.0;')
+    S(f'  for (int i = 0; i < nf; ++i) {{ const int k = {pre}_fsp[{MX} * r + i]; ex -= gl[k]; fwd *= w[k]; }}')
+    S(f'  for (int i = 0; i < nr; ++i) rev *= w[{pre}_rsp[{MX} * r + i]];')
+    S(f'  double F = {pre}_pow2(ex);')
+    S('  if (kind != 0) {')
+    S('    const double M = mt[6] == 3 ? Ms[mt[7]] : (mt[6] == 2 ? Ccol[mt[7]] : Cm);')
+    S(f'    if (kind == {M.THREE_BODY}) {{')
+    S('      F *= M;')
+    S('    } else {')
+    if mixed:
+        S('      // clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32')
+        S(f'      const float Pr = (float)fmin(M * {pre}_pow2(dp[3] + dp[4] * log2T - dp[5] * rcpT), 1e30);')
+        S('      const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
+        S('      float Fc = 1.0f;')
+        S(f'      if (kind == {M.TROE}) {{')
+        S('        const float A_ = (float)dp[6], rT3 = (float)dp[7], rT1 = (float)dp[8];')
+        S('        const float term3 = dp[9] >= 0.0 ? __builtin_amdgcn_exp2f((float)(-dp[9] * 1.4426950408889634) * rcpTf) : 0.0f;')
+        S('        const float logFc = 0.3010299956639812f * __builtin_amdgcn_logf((1.0f - A_) * '
+          '__builtin_amdgcn_exp2f(-rT3 * 1.4426950408889634f * Tf) + A_ * __builtin_amdgcn_exp2f(-rT1 * 1.4426950408889634f * Tf) + term3);')
+        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+        S('        const float tc = -0.4f - 0.67f * logFc + logPr;')
+        S('        const float tf = tc * __builtin_amdgcn_rcpf(0.75f - 1.27f * logFc - 0.14f * tc + 1e-37f);')
+        S('        Fc = __builtin_amdgcn_exp2f(3.3219280948873622f * logFc * __builtin_amdgcn_rcpf(1.0f + tf * tf));')
+        S(f'      }} else if (kind == {M.SRI}) {{')
+        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+        S('        Fc = (float)dp[9] * powf(Tf, (float)dp[10]) * powf((float)dp[6] * expf((float)(-dp[7]) * rcpTf) + '
+          'expf((float)(-dp[8]) * Tf), __builtin_amdgcn_rcpf(1.0f + logPr * logPr));')
+        S('      }')
+        S('      F *= (double)(Pr1 * Fc);')
+    else:
+        S(f'      const double Pr = M * {pre}_pow2(dp[3] + dp[4] * log2T - dp[5] * rcpT);')
+        S('      const double Pr1 = Pr / (1.0 + Pr);')
+        S('      double Fc = 1.0;')
+        S(f'      if (kind == {M.TROE}) {{')
+        S('        const double term3 = dp[9] >= 0.0 ? exp(-dp[9] * rcpT) : 0.0;')
+        S('        const double logFc = log10((1.0 - dp[6]) * exp(-dp[7] * T) + dp[6] * exp(-dp[8] * T) + term3);')
+        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
+        S('        const double tc = -0.4 - 0.67 * logFc + logPr;')
+        S(f'        const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {E.CFLOAT_MIN});')
+        S('        Fc = exp10(logFc / (1.0 + tf * tf));')
+        S(f'      }} else if (kind == {M.SRI}) {{')
+        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
+        S('        Fc = dp[9] * pow(T, dp[10]) * pow(dp[6] * exp(-dp[7] * rcpT) + exp(-dp[8] * T), 1.0 / (1.0 + logPr * logPr));')
+        S('      }')
+        S('      F *= Pr1 * Fc;')
+    S('    }')
+    S('  }')
+    S('  double q;')
+    S('  if (mt[1]) {')
+    S('    double c0 = 1.0;')
+    S('    const int dnu = mt[5];')
+    S('    for (int i = 0; i < (dnu < 0 ? -dnu : dnu); ++i) c0 *= dnu < 0 ? C0 : rcpC0;')
+    S('    q = F * (fwd - c0 * rev);')
+    S('  } else {')
+    S('    q = F * fwd;')
+    S('  }')
+    S(f'  for (int i = 0; i < mt[4]; ++i) wd[{pre}_nsp[{MS} * r + i]] += {pre}_nnu[{MS} * r + i] * q;')
+    S('}')
+    S('// ---- mass production rates and heat release rate')
+    S('double sum_h = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{')
+    S(f'  rates[id + offsetT + k * offset] = {pre}_W[k] * wd[k];')
+    S(f'  const double* a = (T > {pre}_tmid[k] ? {pre}_hhi : {pre}_hlo) + 6 * k;')
+    S('  sum_h += wd[k] * (a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4 + a[5] * rcpT);')
+    S('}')
+    S(f'rates[id] = {repr(-float(t["R"]))} * T * sum_h;')
+    S('}', 0)
+    return S.text()

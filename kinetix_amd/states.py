@@ -49,6 +49,40 @@ def sample(n, species, W, T_range=(1000.0, 2500.0), seed=BASE_SEED, p_range_atm=
     return state, p
 
 
+def reference_mass_fractions(species, W):
+    """The reference composition of kx_build: stoichiometric CH4/air where the mechanism
+    has those species, else equal mass fractions."""
+    try:
+        return air_mass_fractions(species, W)
+    except ValueError:
+        return np.full(len(species), 1.0 / len(species))
+
+
+def sample_any(n, species, W, T_range=(1000.0, 2500.0), seed=BASE_SEED, p_range_atm=None):
+    """sample() for any mechanism: the CH4/air blend where it exists, else Dirichlet(0.5)
+    alone (synthetic mechanisms have no CH4/O2/N2)."""
+    try:
+        air_mass_fractions(species, W)
+        mix = 0.5
+    except ValueError:
+        mix = 0.0
+    if mix:
+        return sample(n, species, W, T_range=T_range, seed=seed, p_range_atm=p_range_atm)
+    rng = np.random.default_rng(seed)
+    N = len(species)
+    T = rng.uniform(T_range[0], T_range[1], size=n)
+    G = rng.standard_gamma(0.5, size=(N, n))
+    G /= G.sum(axis=0, keepdims=True)
+    state = np.empty((N + 1, n))
+    state[0] = T
+    state[1:] = G
+    p = None
+    if p_range_atm is not None:
+        lo, hi = np.log(p_range_atm[0]), np.log(p_range_atm[1])
+        p = ONE_ATM * np.exp(rng.uniform(lo, hi, size=n))
+    return state, p
+
+
 def product_mass_fractions(species, W):
     """Complete-combustion products of stoichiometric CH4/air: CO2 + 2 H2O + 7.52 N2."""
     X = np.zeros(len(species))
