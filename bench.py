@@ -126,7 +126,13 @@ def main():
     N, n = kx.nSpecies(), a.states
 
     t0 = time.time()
-    st, _ = states.sample_any(n, sp, W, T_range=(1000.0, 2500.0), seed=states.BASE_SEED + 2)
+    near_eq = None
+    if 'CH4' in sp and 'O2' in sp and 'N2' in sp:
+        # the mixed-precision workload: 10 % of the states within 1 % of chemical equilibrium
+        st, near_eq = states.sample_near_equilibrium_mix(n, sp, W, t, T_range=(1000.0, 2500.0),
+                                                         seed=states.BASE_SEED + 2)
+    else:
+        st, _ = states.sample_any(n, sp, W, T_range=(1000.0, 2500.0), seed=states.BASE_SEED + 2)
     gen_s = time.time() - t0
     dev = torch.device('cuda')
     d_state = torch.from_numpy(st).to(dev).reshape(-1)
@@ -278,6 +284,25 @@ def main():
     wref = ref[1:]
     l2 = [float(np.linalg.norm(rates[1 + k] - wref[k]) / np.linalg.norm(wref[k]))
           for k in range(N) if np.linalg.norm(wref[k]) > 0]
+    if near_eq is not None:
+        # the cancellation stress: errors on the near-equilibrium states alone, and how far
+        # from equilibrium they are (|q_net| / q_fwd over reversible reactions)
+        ne = np.nonzero(near_eq)[0][:a.err_sample]
+        s_ne = np.ascontiguousarray(st[:, ne])
+        ref_ne, sc_ne = orc.production_rates(s_ne, P_ATM, scale=True, threads=cores)
+        q_ne, qf_ne = orc.reaction_rates(s_ne, P_ATM, forward=True)
+        revm = np.asarray(t['reversible']).astype(bool)
+        rq = np.abs(q_ne[revm]) / np.maximum(qf_ne[revm], 1e-300)
+        rq = rq[qf_ne[revm] > 1e-200]
+        out_ne = d_rates.reshape(N + 1, n)[:, torch.from_numpy(ne).to(dev)].cpu().numpy()
+        l2_ne = [float(np.linalg.norm(out_ne[1 + k] - ref_ne[1 + k]) / np.linalg.norm(ref_ne[1 + k]))
+                 for k in range(N) if np.linalg.norm(ref_ne[1 + k]) > 0]
+        near_eq_report = {'fraction': float(near_eq.mean()), 'sample_states': int(len(ne)),
+                          'qnet_over_qfwd_quantiles_10_50_90_99': [float(v) for v in np.quantile(rq, [0.1, 0.5, 0.9, 0.99])],
+                          'bk1_max_rel_l2_per_species': max(l2_ne),
+                          'bk1_max_rel_cond_aware': rel(out_ne, ref_ne, sc_ne)}
+    else:
+        near_eq_report = None
     err = {'bk1_max_rel_cond_aware': rel(rates, ref, scale),
            'bk1_max_rel_l2_per_species': max(l2),
            'bk2_conductivity_max_rel': rel(cond, c0), 'bk2_viscosity_max_rel': rel(visc, v0),
@@ -347,7 +372,9 @@ def main():
         'bk2_variants': bk2_variants,
         'data': 'synthetic',
         'config': {'workload': f'{a.mech} BK1 production rates + BK2 mixture-averaged transport, '
-                               f'{n} states, T~U[1000,2500] K, 1 atm, Y=0.5 Dirichlet(0.5)+0.5 CH4/air',
+                               f'{n} states, T~U[1000,2500] K, 1 atm, Y=0.5 Dirichlet(0.5)+0.5 CH4/air'
+                               + (', 10% of states within 1% of chemical equilibrium' if near_eq is not None
+                                  else ''),
                    'mechanism': a.mech, 'n_states': n, 'bk1': a.bk1, 'bk2': a.bk2,
                    'n_species': N, 'n_reactions': kx.nReactions()},
         'roofline': roofline,
@@ -356,6 +383,7 @@ def main():
         'bk1_GRXN_s': round(n * kx.nReactions() / m1 / 1e6, 2),
         'bk2_GDOF_s': round(n * (N + 2) / m2 / 1e6, 2),
         'err_vs_fp64_oracle': err,
+        'near_equilibrium': near_eq_report,
         'state_gen_s': round(gen_s, 2),
     }
     if sur is not None:

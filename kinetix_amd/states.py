@@ -83,6 +83,35 @@ def sample_any(n, species, W, T_range=(1000.0, 2500.0), seed=BASE_SEED, p_range_
     return state, p
 
 
+def sample_near_equilibrium_mix(n, species, W, t, T_range=(1000.0, 2500.0), seed=BASE_SEED,
+                                frac=0.1, rel=0.01, pool=16384):
+    """The mixed-precision workload's states: sample() for all states, then a seeded `frac` of
+    them (at random positions) replaced by states within `rel` of chemical equilibrium:
+    equilibria (kinetix_amd/equilibrium.py) of up to `pool` base states at their own T and
+    1 atm, each reused with an independent per-species perturbation exp(U[-ln(1+rel),
+    ln(1+rel)]). Returns (state, near_eq_mask)."""
+    from . import equilibrium as eq
+    state, _ = sample(n, species, W, T_range=T_range, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    m = int(round(frac * n))
+    pos = np.sort(rng.choice(n, m, replace=False)) if m else np.zeros(0, dtype=np.int64)
+    mask = np.zeros(n, dtype=bool)
+    mask[pos] = True
+    if m:
+        k = min(pool, m)
+        base = pos[:k]
+        Wc = np.asarray(W, dtype=np.float64)[:, None]
+        X = state[1:, base] / Wc
+        X /= X.sum(0, keepdims=True)
+        Xe = eq.equilibrate(t, state[0, base], np.full(k, ONE_ATM), X)
+        pick = rng.integers(0, k, size=m)
+        Xp = Xe[:, pick] * np.exp(rng.uniform(-np.log1p(rel), np.log1p(rel), size=(len(species), m)))
+        Yp = Xp * Wc
+        state[1:, pos] = Yp / Yp.sum(0, keepdims=True)
+        state[0, pos] = state[0, base[pick]]
+    return state, mask
+
+
 def product_mass_fractions(species, W):
     """Complete-combustion products of stoichiometric CH4/air: CO2 + 2 H2O + 7.52 N2."""
     X = np.zeros(len(species))
@@ -115,10 +144,32 @@ def sample_blends(n, species, W, T_range=(300.0, 2500.0), p_range_atm=(0.5, 40.0
     return state, p
 
 
-def sample_transport_set(n, species, W, seed=BASE_SEED):
+def sample_transport_set(n, species, W, seed=BASE_SEED, tables=None, near_eq_frac=0.0,
+                         near_eq_pool=4096):
     """Surrogate training/inference distribution: half Dirichlet/air mixtures with
-    T ~ U[300, 2500] K and p ~ logU[0.5, 40] atm, half reactant/product blends."""
+    T ~ U[300, 2500] K and p ~ logU[0.5, 40] atm, half reactant/product blends; with
+    `tables` and near_eq_frac > 0, that fraction of the states (seeded positions) is
+    replaced by states within 1 % of chemical equilibrium at their own T and p (the
+    equilibria of up to near_eq_pool of them, each reused with independent perturbations)."""
     h = n // 2
     s1, p1 = sample(h, species, W, T_range=(300.0, 2500.0), seed=seed, p_range_atm=(0.5, 40.0))
     s2, p2 = sample_blends(n - h, species, W, seed=seed + 7919)
-    return np.concatenate([s1, s2], axis=1), np.concatenate([p1, p2])
+    state, p = np.concatenate([s1, s2], axis=1), np.concatenate([p1, p2])
+    m = int(round(near_eq_frac * n))
+    if tables is not None and m:
+        from . import equilibrium as eq
+        rng = np.random.default_rng(seed + 104729)
+        pos = rng.choice(n, m, replace=False)
+        k = min(near_eq_pool, m)
+        base = pos[:k]
+        Wc = np.asarray(W, dtype=np.float64)[:, None]
+        X = state[1:, base] / Wc
+        X /= X.sum(0, keepdims=True)
+        Xe = eq.equilibrate(tables, state[0, base], p[base], X)
+        pick = rng.integers(0, k, size=m)
+        Xp = Xe[:, pick] * np.exp(rng.uniform(-np.log1p(0.01), np.log1p(0.01), size=(len(species), m)))
+        Yp = Xp * Wc
+        state[1:, pos] = Yp / Yp.sum(0, keepdims=True)
+        state[0, pos] = state[0, base[pick]]
+        p[pos] = p[base[pick]]
+    return state, p

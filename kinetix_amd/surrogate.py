@@ -135,9 +135,13 @@ class Surrogate:
         self.kx.mixtureAvgTransportPropsPstate(n, offsetT, offset, d_p, d_state, visc, cond, rhoD, dkm)
         return visc, cond, rhoD, dkm
 
+    def _tables(self):
+        from .gen import tables
+        return tables.load(self.kx.dir)[1]
+
     # ---------------------------------------------------------------- training
     def train(self, n_train=1 << 24, epochs=20, batch=8192, lr=2e-3, seed=st.BASE_SEED + 4,
-              log=print, holdout=65536, hull_margin=0.02):
+              log=print, holdout=65536, hull_margin=0.02, near_eq_frac=0.1):
         import torch
         N, dev = self.N, self.dev
         sp, W = self.kx.speciesNames(), self.kx.molar_masses
@@ -151,7 +155,8 @@ class Surrogate:
         chunk = 1 << 21
         for c0 in range(0, n, chunk):
             m = min(chunk, n - c0)
-            s_, p_ = st.sample_transport_set(m, sp, W, seed=seed + c0)
+            s_, p_ = st.sample_transport_set(m, sp, W, seed=seed + c0, tables=self._tables(),
+                                             near_eq_frac=near_eq_frac)
             view[:, c0:c0 + m].copy_(torch.from_numpy(s_))
             d_p[c0:c0 + m].copy_(torch.from_numpy(p_))
         visc, cond, _, dkm = self.plain(n, n, n, d_p, d_state)
@@ -179,7 +184,9 @@ class Surrogate:
         _check(self.L.kx_sur_targets(self.kx.h, n, n, _p(visc), _p(cond), _p(dkm), _p(base),
                                      _p(ost0), _p(tgt), _s()))
         om, osd = tgt[:n_train].mean(0), tgt[:n_train].std(0)
-        osd = torch.clamp(osd, min=1e-6)
+        # floor: a residual whose spread is FP32 rounding (mu, lambda: the baselines are
+        # nearly exact) must not be blown up to unit variance -- that is noise to learn
+        osd = torch.clamp(osd, min=1e-3)
         om[VALID:] = 0.0
         osd[VALID:] = 1.0
         ostat = torch.cat([om, osd]).contiguous()
@@ -222,6 +229,7 @@ class Surrogate:
             hist.append(l_)
             log(f'surrogate: epoch {ep + 1}/{epochs} mse {l_:.3e} ({time.time() - t0:.1f} s)')
         meta = {'n_train': n_train, 'epochs': epochs, 'batch': batch, 'lr': lr, 'seed': seed,
+                'near_eq_frac': near_eq_frac,
                 'loss_history': hist, 'train_seconds': time.time() - t0,
                 'mechanism': os.path.basename(self.kx.dir)}
         self.set_model(params.cpu().numpy(), norm.cpu().numpy(), ostat.cpu().numpy(), meta)

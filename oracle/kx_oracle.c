@@ -183,7 +183,7 @@ static void gibbs_exp(const kxo_mech *m, double lnT, double T, double T2, double
 
 /* Net rate of progress of every reaction (q_r); wdot optional (accumulated, +=). */
 static void net_rates(const kxo_mech *m, double lnT, double T, double T2, double T3, double T4,
-                      double rcpT, double P, double lnP, const double *C, double *q)
+                      double rcpT, double P, double lnP, const double *C, double *q, double *qf)
 {
   const int N = m->n_species;
   double eg[KXO_MAXSP], rcp_eg[KXO_MAXSP];
@@ -202,6 +202,7 @@ static void net_rates(const kxo_mech *m, double lnT, double T, double T2, double
       for (int c = 0; c < nf[k]; ++c) { fwd = first_f ? C[k] : fwd * C[k]; first_f = 0; }
       for (int c = 0; c < nr[k]; ++c) { rev = first_r ? C[k] : rev * C[k]; first_r = 0; }
     }
+    if (qf) qf[r] = kf * fwd;
     if (!m->reversible[r]) { q[r] = kf * fwd; continue; }
     /* reverse coefficient = 1/Kc: product of exp(g) over products, 1/exp(g) over
      * reactants (species-index order), times C0^-dnu (reaction_rates.py:1212-1231) */
@@ -223,7 +224,7 @@ void kxo_species_rates(const kxo_mech *m, double lnT, double T, double T2, doubl
 {
   const int N = m->n_species;
   double *q = (double *)malloc(sizeof(double) * (size_t)m->n_reactions);
-  net_rates(m, lnT, T, T2, T3, T4, rcpT, P, lnP, C, q);
+  net_rates(m, lnT, T, T2, T3, T4, rcpT, P, lnP, C, q, NULL);
   for (int r = 0; r < m->n_reactions; ++r) {
     const int *nf = m->nu_f + (size_t)r * N, *nr = m->nu_r + (size_t)r * N;
     for (int k = 0; k < N; ++k) {
@@ -348,7 +349,7 @@ void kxo_production_rates(const kxo_mech *m, int n, int offsetT, int offset, dou
     const double Mbar = 1 / rcpMbar;
     const double rho = pressure_R * rcpT * Mbar;
     for (int k = 0; k < N; ++k) { wrk1[k] = wrk1[k] * rho; wrk2[k] = 0; sc[k] = 0; }
-    net_rates(m, logT, T, T2, T3, T4, rcpT, P, logP, wrk1, q);
+    net_rates(m, logT, T, T2, T3, T4, rcpT, P, logP, wrk1, q, NULL);
     for (int r = 0; r < m->n_reactions; ++r) {
       const int *nf = m->nu_f + (size_t)r * N, *nr = m->nu_r + (size_t)r * N;
       for (int k = 0; k < N; ++k) {
@@ -378,11 +379,14 @@ void kxo_production_rates(const kxo_mech *m, int n, int offsetT, int offset, dou
 /* Net rates of progress q_r [mol/m^3/s] of every reaction at each state (R x n, row-major by
  * reaction), for diagnostics and the equilibrium tests. */
 void kxo_reaction_rates(const kxo_mech *m, int n, int offsetT, int offset, double pressure_,
-                        const double *p_state, const double *state, double *q_out, double Tref)
+                        const double *p_state, const double *state, double *q_out, double *qf_out,
+                        double Tref)
 {
+  /* net rates q_r (and, if qf_out, the forward rates k_f prod C^nu') per reaction */
   const int N = m->n_species;
   double wrk1[KXO_MAXSP];
   double *q = (double *)malloc(sizeof(double) * (size_t)(m->n_reactions + 1));
+  double *qf = qf_out ? (double *)malloc(sizeof(double) * (size_t)(m->n_reactions + 1)) : NULL;
   for (int id = 0; id < n; ++id) {
     const double P = p_state ? p_state[id] : pressure_;
     const double T = Tref * state[id];
@@ -391,10 +395,14 @@ void kxo_reaction_rates(const kxo_mech *m, int n, int offsetT, int offset, doubl
     state_prologue(m, id, offsetT, offset, state, wrk1, &rcpMbar);
     const double rho = P / m->R * rcpT / rcpMbar;
     for (int k = 0; k < N; ++k) wrk1[k] *= rho;
-    net_rates(m, log(T), T, T * T, T * T * T, T * T * T * T, rcpT, P, log(P), wrk1, q);
-    for (int r = 0; r < m->n_reactions; ++r) q_out[(size_t)r * n + id] = q[r];
+    net_rates(m, log(T), T, T * T, T * T * T, T * T * T * T, rcpT, P, log(P), wrk1, q, qf);
+    for (int r = 0; r < m->n_reactions; ++r) {
+      q_out[(size_t)r * n + id] = q[r];
+      if (qf_out) qf_out[(size_t)r * n + id] = qf[r];
+    }
   }
   free(q);
+  free(qf);
 }
 
 /* transportProps.okl:11-47; pressure is the value the reference passes (normalised p, it

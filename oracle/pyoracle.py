@@ -91,7 +91,7 @@ class Oracle:
         L.kxo_thermo.argtypes = [C.POINTER(_Mech), C.c_int, C.c_int, C.c_int, C.c_double,
                                  _dp, _dp, _dp, _dp, C.c_double]
         L.kxo_reaction_rates.argtypes = [C.POINTER(_Mech), C.c_int, C.c_int, C.c_int, C.c_double,
-                                         _dp, _dp, _dp, C.c_double]
+                                         _dp, _dp, _dp, _dp, C.c_double]
 
     # Threads: each call works on a contiguous chunk of states (ctypes releases the GIL).
     @staticmethod
@@ -149,14 +149,16 @@ class Oracle:
                             _ptr(state), _ptr(rho), _ptr(cp), _ptr(rhocp), float(Tref))
         return rho, cp, rhocp
 
-    def reaction_rates(self, state, p, Tref=1.0, p_state=None):
+    def reaction_rates(self, state, p, Tref=1.0, p_state=None, forward=False):
+        """Net rates q_r (R, n); with forward=True also the forward rates k_f prod C^nu'."""
         state = np.ascontiguousarray(state, dtype=np.float64)
         n = state.shape[1]
         q = np.zeros((self.R, n))
+        qf = np.zeros((self.R, n)) if forward else None
         ps = np.ascontiguousarray(p_state, dtype=np.float64) if p_state is not None else None
         self.lib.kxo_reaction_rates(C.byref(self.m), n, n, n, float(p), _ptr(ps), _ptr(state),
-                                    _ptr(q), float(Tref))
-        return q
+                                    _ptr(q), _ptr(qf), float(Tref))
+        return (q, qf) if forward else q
 
     def _run(self, n, threads, fn):
         if threads <= 1:

@@ -86,3 +86,18 @@ def test_mixed_subview(kx, golden, accum):
     # a state's result depends only on that state (tile-mates only share the MFMA)
     np.testing.assert_allclose(part[:, lo:lo + m], full[:, lo:lo + m], rtol=1e-6, atol=0)
     assert np.all(np.isnan(part[:, :lo])) and np.all(np.isnan(part[:, lo + m:]))
+
+
+@pytest.mark.parametrize('accum', ACCUM)
+def test_mixed_near_equilibrium(kx, oracle, gri, accum):
+    """States within 1 % of chemical equilibrium (net rates cancel to ~1 % of the forward
+    rates): the FP64 K_c and net subtraction keep the error within the same bound."""
+    info, t = gri
+    s, mask = states.sample_near_equilibrium_mix(40960, info['species'], t['W'], t,
+                                                  seed=states.BASE_SEED + 2, pool=1024)
+    s = np.ascontiguousarray(s[:, mask])
+    out = run(kx, s, P_ATM, mixed=accum)
+    ref, scale = oracle.production_rates(s, P_ATM, scale=True, threads=8)
+    e = l2_per_species(out[1:], ref[1:])
+    assert e.max() < L2_BOUND, e.max()
+    assert cond_err(out, ref, scale) < 2e-5
