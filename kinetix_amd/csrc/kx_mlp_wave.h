@@ -13,7 +13,8 @@
 //    permutation, so activations never leave registers between layers;
 //  * weights stream from L2 as pre-packed fragments (one fragment = 64 lanes x 8 bf16 =
 //    1 KB, contiguous), each reused for the wave's 4 column blocks of 16 states, requested
-//    one output block ahead of their use;
+//    ahead of their use; the hidden layers are software-pipelined so one output block's
+//    MFMAs overlap the previous block's GELU;
 //  * bias is the accumulator's initial value; GELU (tanh form, written as x * sigmoid(2u))
 //    costs one exp2 and one reciprocal; the results are packed to BF16 with v_cvt_pk_bf16_f32.
 #pragma once
@@ -95,42 +96,63 @@ __device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32>& in
     }
 }
 
-// The KB fragments and the bias of output block ob + 1 are requested before the MFMAs of
-// block ob (bias first: vmcnt retires loads in order, so no wait for the current block's
-// operands can cover the next block's), so their L2 latency overlaps a block's worth of
-// MFMA and GELU work.
+// Software-pipelined over output blocks with two accumulator sets used alternately (no
+// copies between them): while the MFMAs of block ob + 1 run, bias + GELU is applied to block
+// ob, so the MFMA pipe and the VALU overlap; the fragments and bias of block ob + 2 are
+// requested one block ahead (bias first: vmcnt retires loads in order).
+template <int KB>
+__device__ __forceinline__ void mfma_block(const u32x4 (&a)[KB], const f32x4& b, const Act<KB>& in,
+                                           f32x4 (&acc)[4])
+{
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) acc[sb] = b;
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a[kb], in[kb][sb], acc[sb]);
+}
+
+__device__ __forceinline__ void gelu_block(const f32x4 (&acc)[4], Act<8>& out, int ob)
+{
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) {
+    const unsigned lo = pack2(gelu_fast(acc[sb][0]), gelu_fast(acc[sb][1]));
+    const unsigned hi = pack2(gelu_fast(acc[sb][2]), gelu_fast(acc[sb][3]));
+    out[ob >> 1][sb][(ob & 1) * 2] = lo;
+    out[ob >> 1][sb][(ob & 1) * 2 + 1] = hi;
+  }
+}
+
+template <int KB>
+__device__ __forceinline__ void load_block(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+                                           int ob, int lane, u32x4 (&a)[KB], f32x4& b)
+{
+  b = *(const f32x4*)(bias + 16 * ob + 4 * (lane >> 4));
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) a[kb] = wf[(ob * KB + kb) * 64 + lane];
+}
+
 template <int KB>
 __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
                                              const Act<KB>& in, Act<8>& out, int lane)
 {
-  const int g = lane >> 4;
-  u32x4 a_cur[KB], a_nxt[KB];
-  f32x4 b_cur = *(const f32x4*)(bias + 4 * g), b_nxt;
+  constexpr int OB = kH / 16;
+  u32x4 a0[KB], a1[KB];
+  f32x4 b0, b1;
+  f32x4 accA[4], accB[4];
+  load_block<KB>(wf, bias, 0, lane, a0, b0);
+  load_block<KB>(wf, bias, 1, lane, a1, b1);
+  mfma_block<KB>(a0, b0, in, accA);                       // block 0
 #pragma unroll
-  for (int kb = 0; kb < KB; ++kb) a_cur[kb] = wf[kb * 64 + lane];
-#pragma unroll
-  for (int ob = 0; ob < kH / 16; ++ob) {
-    if (ob + 1 < kH / 16) {
-      b_nxt = *(const f32x4*)(bias + 16 * (ob + 1) + 4 * g);
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) a_nxt[kb] = wf[((ob + 1) * KB + kb) * 64 + lane];
-    }
-    __builtin_amdgcn_sched_barrier(0);      // keep the requests here, ahead of the MFMAs
-    f32x4 acc[4] = {b_cur, b_cur, b_cur, b_cur};
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-      for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a_cur[kb], in[kb][sb], acc[sb]);
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb) {
-      const unsigned lo = pack2(gelu_fast(acc[sb][0]), gelu_fast(acc[sb][1]));
-      const unsigned hi = pack2(gelu_fast(acc[sb][2]), gelu_fast(acc[sb][3]));
-      out[ob >> 1][sb][(ob & 1) * 2] = lo;
-      out[ob >> 1][sb][(ob & 1) * 2 + 1] = hi;
-    }
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) a_cur[kb] = a_nxt[kb];
-    b_cur = b_nxt;
+  for (int ob = 0; ob < OB; ob += 2) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (ob + 2 < OB) load_block<KB>(wf, bias, ob + 2, lane, a0, b0);
+    mfma_block<KB>(a1, b1, in, accB);                     // block ob + 1 ...
+    gelu_block(accA, out, ob);                            // ... overlaps block ob's GELU
+    __builtin_amdgcn_sched_barrier(0);
+    if (ob + 3 < OB) load_block<KB>(wf, bias, ob + 3, lane, a1, b1);
+    if (ob + 2 < OB) mfma_block<KB>(a0, b0, in, accA);    // block ob + 2 ...
+    gelu_block(accB, out, ob + 1);                        // ... overlaps block ob + 1's GELU
   }
 }
 
