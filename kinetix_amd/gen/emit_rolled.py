@@ -23,6 +23,8 @@ from . import emit_hip as E
 from . import mechanism as M
 
 ROLL_N = 96          # mechanisms with more species than this get the loop forms
+BK2_BLOCK = 8        # j-species held in registers by the blocked BK2 pair loops (8: 2 waves/SIMD)
+BK2_KUNROLL = 2      # k-loop unroll (overlaps the next pair coefficients' loads)
 
 
 def _table(name, arr, ctype='double'):
@@ -54,6 +56,14 @@ def emit_bk2(t, info, pstate=False):
     S(_table(f'{pre}_diff', pairs), 0)
     S(_table(f'{pre}_C1', t['C1']), 0)
     S(_table(f'{pre}_C2', t['C2']), 0)
+    S('// 1/d for the positive pair fits: hardware reciprocal + two Newton steps (~1 ulp,')
+    S('// no special cases), in place of the IEEE divide sequence')
+    S(f'__device__ __forceinline__ double {pre}_rcp64(double d)', 0)
+    S('{', 0)
+    S('double r = __builtin_amdgcn_rcp(d);')
+    S('r = fma(r, fma(-d, r, 1.0), r);')
+    S('return fma(r, fma(-d, r, 1.0), r);')
+    S('}', 0)
     S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
     if pstate:
         S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
@@ -86,23 +96,42 @@ def emit_bk2(t, info, pstate=False):
     S('const double Mbar = 1.0 / rcpMbar;')
     S('#pragma unroll 1')
     S(f'for (int k = 0; k < {N}; ++k) {{ x[k] *= Mbar; s[k] = 0.0; }}')
-    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj')
-    S('int pr = 0;')
-    S('#pragma unroll 1')
-    S(f'for (int k = 1; k < {N}; ++k) {{')
-    S('  const double xk = x[k];')
-    S('  double sk = s[k];')
-    S('  #pragma unroll 2')
-    S('  for (int j = 0; j < k; ++j, ++pr) {')
-    if rcp_diff:
-        S(f'    const double rD = {_poly(pre + "_diff", "pr")};')
-    else:
-        S(f'    const double rD = 1.0 / {_poly(pre + "_diff", "pr")};')
-    S('    sk += x[j] * rD;')
-    S('    s[j] += xk * rD;')
-    S('  }')
-    S('  s[k] = sk;')
-    S('}')
+    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj,')
+    S('// register-blocked: a block of 16 j-species (x_j and their S_j sums) stays in registers')
+    S('// while k runs over the later species (one private load and one update per k)')
+    JB = BK2_BLOCK
+
+    def rd(pidx):
+        if rcp_diff:
+            return _poly(pre + '_diff', pidx)
+        return f'{pre}_rcp64({_poly(pre + "_diff", pidx)})'
+    for j0 in range(0, N, JB):
+        nb = min(JB, N - j0)
+        S('{')
+        for i in range(nb):
+            S(f'  const double xj{i} = x[{j0 + i}];')
+            S(f'  double sj{i} = 0.0;')
+        # diagonal block: pairs (k, j) with j0 <= j < k < j0 + nb, unrolled
+        for kk in range(1, nb):
+            k = j0 + kk
+            S(f'  {{ double sk = 0.0;')
+            for i in range(kk):
+                pidx = k * (k - 1) // 2 + j0 + i
+                S(f'    {{ const double r_ = {rd(str(pidx))}; sk += xj{i} * r_; sj{i} += xj{kk} * r_; }}')
+            S(f'    sj{kk} += sk; }}')
+        if j0 + nb < N:
+            S(f'  #pragma unroll {BK2_KUNROLL}')
+            S(f'  for (int k = {j0 + nb}; k < {N}; ++k) {{')
+            S('    const double xk = x[k];')
+            S(f'    const int pb = k * (k - 1) / 2 + {j0};')
+            S('    double sk = 0.0;')
+            for i in range(nb):
+                S(f'    {{ const double r_ = {rd(f"pb + {i}")}; sk += xj{i} * r_; sj{i} += xk * r_; }}')
+            S('    s[k] += sk;')
+            S('  }')
+        for i in range(nb):
+            S(f'  s[{j0 + i}] += sj{i};')
+        S('}')
     S('const double TsqrT = T * sqrT;')
     S(f'const double rho = p * {repr(1.0 / float(t["R"]))} * rcpT * Mbar;')
     S('const double pMbar = p * Mbar;')
@@ -113,21 +142,29 @@ def emit_bk2(t, info, pstate=False):
     if pstate:
         S('  Dkm[k * offset + id] = D;')
     S('}')
-    S('// ---- viscosity (Wilke)')
+    S('// ---- viscosity (Wilke), register-blocked over 16 j-species: S_k = sum_j X_j Phi_kj')
+    S(f'double sv[{N}], ev[{N}];')
     S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) re[k] = 1.0 / {_poly(pre + "_visc", "k")};')
+    S(f'for (int k = 0; k < {N}; ++k) {{ const double e = {_poly(pre + "_visc", "k")}; ev[k] = e; re[k] = 1.0 / e; sv[k] = 0.0; }}')
+    for j0 in range(0, N, JB):
+        nb = min(JB, N - j0)
+        S('{')
+        for i in range(nb):
+            S(f'  const double xw{i} = x[{j0 + i}], rw{i} = re[{j0 + i}];')
+        S(f'  #pragma unroll {BK2_KUNROLL}')
+        S(f'  for (int k = 0; k < {N}; ++k) {{')
+        S('    const double e = ev[k];')
+        S(f'    const double* c1 = {pre}_C1 + k * {N} + {j0};')
+        S(f'    const double* c2 = {pre}_C2 + k * {N} + {j0};')
+        S('    double acc = 0.0;')
+        for i in range(nb):
+            S(f'    {{ const double f = c1[{i}] + c2[{i}] * e * rw{i}; acc += xw{i} * f * f; }}')
+        S('    sv[k] += acc;')
+        S('  }')
+        S('}')
     S('double vis = 0.0;')
     S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) {{')
-    S(f'  const double e = {_poly(pre + "_visc", "k")};')
-    S('  double sv = 0.0;')
-    S('  #pragma unroll 4')
-    S(f'  for (int j = 0; j < {N}; ++j) {{')
-    S(f'    const double f = {pre}_C1[k * {N} + j] + {pre}_C2[k * {N} + j] * e * re[j];')
-    S('    sv += x[j] * f * f;')
-    S('  }')
-    S('  vis += x[k] * e * e / sv;')
-    S('}')
+    S(f'for (int k = 0; k < {N}; ++k) vis += x[k] * ev[k] * ev[k] / sv[k];')
     S('viscosity[id] = sqrT * vis;')
     S('// ---- conductivity')
     S('double sum1 = 0.0, sum2 = 0.0;')
