@@ -366,6 +366,16 @@ class _RateEmitter:
             parts = ['0.0']
         return ' + '.join(parts) + extra
 
+    def _log2_rate32(self, A, beta, Ta):
+        """FP32 expression of log2(A T^beta exp(-Ta/T)) with inline literal constants."""
+        L2E = 1.4426950408889634
+        parts = [f'{float(math.log2(A))!r}f'] if A != 1.0 else []
+        if float(beta) != 0.0:
+            parts.append(f'{float(beta)!r}f * log2Tf')
+        if float(Ta) != 0.0:
+            parts.append(f'{float(-float(Ta) * L2E)!r}f * rcpTf')
+        return ' + '.join(parts) if parts else '0.0f'
+
     def emit_reaction_w(self, S, r, sidx, pow2='kx_pow2', fp64_falloff=False):
         """Emit `double q<r>` in the w-form. The scale factor k_f exp(-sum g_r) comes from
         one pow2() of an FP64 exponent (the Gibbs sum folded into the Arrhenius exponent);
@@ -377,7 +387,12 @@ class _RateEmitter:
         S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
         if kind == M.PLOG:
             raise NotImplementedError('PLOG kernels are not emitted yet')
-        S(f'  double F = {pow2}({self._log2_rate(t["A"][r], t["beta"][r], t["Ta"][r], gsum)});')
+        if getattr(self, 'arrh32', False) and not fp64_falloff:
+            # Arrhenius exponent in FP32 (inline literals); the Gibbs sum stays FP64
+            S(f'  const float ya = {self._log2_rate32(t["A"][r], t["beta"][r], t["Ta"][r])};')
+            S(f'  double F = {pow2}((double)ya{gsum});')
+        else:
+            S(f'  double F = {pow2}({self._log2_rate(t["A"][r], t["beta"][r], t["Ta"][r], gsum)});')
         if kind == M.THREE_BODY:
             S(f'  F *= {self._third_w(r, sidx)};')
         elif kind in (M.LINDEMANN, M.TROE, M.SRI) and fp64_falloff:
@@ -406,9 +421,14 @@ class _RateEmitter:
             ratio = math.log2(t['A0'][r] / t['A'][r])
             db = float(t['beta0'][r]) - float(t['beta'][r])
             dT = float(t['Ta0'][r]) - float(t['Ta'][r])
-            # clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32
-            S(f'  const float Pr = (float)fmin({self._third_w(r, sidx)} * '
-              f'{pow2}({self._log2_rate(2.0 ** ratio, db, dT)}), 1e30);')
+            # clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32; the FP32
+            # exponent is capped at 100 so exp2f stays finite (no 0 * inf at M = 0)
+            if getattr(self, 'arrh32', False):
+                S(f'  const float Pr = fminf((float){self._third_w(r, sidx)} * '
+                  f'__builtin_amdgcn_exp2f(fminf({self._log2_rate32(2.0 ** ratio, db, dT)}, 100.0f)), 1e30f);')
+            else:
+                S(f'  const float Pr = (float)fmin({self._third_w(r, sidx)} * '
+                  f'{pow2}({self._log2_rate(2.0 ** ratio, db, dT)}), 1e30);')
             S('  const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
             if kind == M.LINDEMANN:
                 S('  F *= (double)Pr1;')
@@ -508,14 +528,16 @@ class _RateEmitter:
         return '(' + '*'.join(terms) + ')' if terms else '1.0'
 
 
-def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64'):
+def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64', arrh32=False):
     """variant 'fp64': plain FP64 kernel. variant 'mixed_valu': the mixed-precision rate
-    coefficients (FP32 exp2 with FP64 scaling, FP32 falloff) with FP64 VALU accumulation of
-    wdot instead of MFMA (w-form only)."""
+    coefficients (FP32 exp2 with FP64 scaling, FP32 falloff; with arrh32 the Arrhenius
+    exponent too is formed in FP32, then added to the FP64 Gibbs sum) with FP64 VALU
+    accumulation of wdot instead of MFMA (w-form only)."""
     N = int(t['n_species'])
     em = _RateEmitter(t, info)
     em.sched_window = sched_window
     mixed = variant == 'mixed_valu'
+    em.arrh32 = mixed and arrh32
     if mixed:
         wform = True
     S = _Src()
@@ -544,7 +566,8 @@ def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64'):
     S(f'const double log2T = lnT * {lit(1.4426950408889634)};')
     S('const float Tf = (float)T;')
     S('const float rcpTf = (float)rcpT;')
-    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf;')
+    S('const float log2Tf = (float)log2T;')
+    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
     S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
     rcpW = t['rcpW']
     for k in range(N):
@@ -768,7 +791,7 @@ def _with_pool(name, fn, *args, lds=False, block=BLOCK):
 
 
 DEFAULT_OPTS = {'bk1_sched_window': 0, 'bk1_wform': True, 'mixed': None, 'lds_pool': True,
-                'mixed_lds_pool': False}
+                'mixed_lds_pool': False, 'mixed_arrh32': True}
 
 
 def generate(tables, info, opts=None):
@@ -796,7 +819,7 @@ def generate(tables, info, opts=None):
     parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],
                             lds=o['mixed_lds_pool']))
     parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_fp64, tables, info, 0, True, 'mixed_valu',
-                            lds=o['lds_pool']))
+                            o['mixed_arrh32'], lds=o['lds_pool']))
     if 'cond' in tables:
         from . import emit_surrogate
         parts.append(_with_pool('kx_sur_k', emit_surrogate.emit, tables, info))

@@ -31,6 +31,8 @@ def _table(name, arr, ctype='double'):
     a = np.asarray(arr).reshape(-1)
     if ctype == 'double':
         body = ', '.join(repr(float(v)) for v in a)
+    elif ctype == 'float':
+        body = ', '.join(f'{float(np.float32(v))!r}f' for v in a)
     else:
         body = ', '.join(str(int(v)) for v in a)
     return f'__device__ const {ctype} {name}[{max(a.size, 1)}] = {{{body if a.size else "0"}}};'
@@ -240,7 +242,9 @@ def _bk1_tables(t, pre):
         if kind[r] == M.PLOG:
             raise NotImplementedError('PLOG kernels are not emitted yet')
         dbl[r] = row
-    out = [_table(f'{pre}_fsp', f_sp, 'int'), _table(f'{pre}_rsp', r_sp, 'int'),
+    # FP32 copies of the Arrhenius and low-pressure exponent coefficients (mixed variant)
+    flt = dbl[:, :6].astype(np.float32)
+    out = [_table(f'{pre}_flt', flt, 'float'),_table(f'{pre}_fsp', f_sp, 'int'), _table(f'{pre}_rsp', r_sp, 'int'),
            _table(f'{pre}_nsp', n_sp, 'int'), _table(f'{pre}_nnu', n_nu),
            _table(f'{pre}_meta', meta, 'int'), _table(f'{pre}_dbl', dbl),
            _table(f'{pre}_setoff', np.array(set_off), 'int'), _table(f'{pre}_setsp', np.array(set_sp), 'int'),
@@ -253,8 +257,9 @@ def _bk1_tables(t, pre):
 
 def emit_bk1(t, info, variant='fp64'):
     """Rolled BK1. variant 'fp64': FP64 exponentials and falloff; 'mixed_valu': Arrhenius
-    through kx_pow2 (FP64 exponent, FP32 exp2 of its fraction), falloff in FP32, Gibbs
-    terms, net rates and the wdot sums in FP64 -- the straight-line forms' split."""
+    exponent in FP32, its sum with the FP64 Gibbs terms through kx_pow2 (FP64 rint, FP32
+    exp2 of the fraction), falloff in FP32; Gibbs terms, net rates and the wdot sums in
+    FP64 -- the straight-line forms' split."""
     N, R = int(t['n_species']), int(t['n_reactions'])
     name = f'kx_bk1_{variant}'
     pre = f'{name}_r'
@@ -285,8 +290,8 @@ def emit_bk1(t, info, variant='fp64'):
     S('const double lnT = log(T);')
     S('const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;')
     S('const double log2T = lnT * 1.4426950408889634;')
-    S('const float Tf = (float)T, rcpTf = (float)rcpT;')
-    S('(void)Tf; (void)rcpTf;')
+    S('const float Tf = (float)T, rcpTf = (float)rcpT, log2Tf = (float)log2T;')
+    S('(void)Tf; (void)rcpTf; (void)log2Tf;')
     S(f'double w[{N}], gl[{N}], wd[{N}], Ms[{max(nsets, 1)}];   // per-lane private arrays')
     S('double rcpMbar = 0.0;')
     S('#pragma unroll 1')
@@ -319,7 +324,11 @@ def emit_bk1(t, info, variant='fp64'):
     S(f'  const int* mt = {pre}_meta + 8 * r;')
     S(f'  const double* dp = {pre}_dbl + 13 * r;')
     S('  const int kind = mt[0], nf = mt[2], nr = mt[3];')
-    S('  double ex = dp[0] + dp[1] * log2T - dp[2] * rcpT;')
+    if mixed:
+        S(f'  const float* fp = {pre}_flt + 6 * r;')
+        S('  double ex = (double)(fp[0] + fp[1] * log2Tf - fp[2] * rcpTf);   // Arrhenius exponent in FP32')
+    else:
+        S('  double ex = dp[0] + dp[1] * log2T - dp[2] * rcpT;')
     S('  double fwd = 1.0, rev = 1.0;')
     S(f'  for (int i = 0; i < nf; ++i) {{ const int k = {pre}_fsp[{MX} * r + i]; ex -= gl[k]; fwd *= w[k]; }}')
     S(f'  for (int i = 0; i < nr; ++i) rev *= w[{pre}_rsp[{MX} * r + i]];')
@@ -331,7 +340,8 @@ def emit_bk1(t, info, variant='fp64'):
     S('    } else {')
     if mixed:
         S('      // clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32')
-        S(f'      const float Pr = (float)fmin(M * {pre}_pow2(dp[3] + dp[4] * log2T - dp[5] * rcpT), 1e30);')
+        S('      // the FP32 exponent is capped at 100 so exp2f stays finite (no 0 * inf at M = 0)')
+        S('      const float Pr = fminf((float)M * __builtin_amdgcn_exp2f(fminf(fp[3] + fp[4] * log2Tf - fp[5] * rcpTf, 100.0f)), 1e30f);')
         S('      const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
         S('      float Fc = 1.0f;')
         S(f'      if (kind == {M.TROE}) {{')
