@@ -16,7 +16,9 @@ Design (MI355X-first, not a translation of the reference's rolled C):
   * the mechanism is straight-line code: every table index is a compile-time constant, so
     per-state arrays (concentrations, Gibbs exponentials, wdot) live in registers, not in
     scratch, and no rate constant / q_r array is ever materialised;
-  * rate constants sharing (beta, E/R) share one exp; reactions are emitted grouped by it;
+  * BK1 (w-form) is emitted in a register-liveness order (liveness.py): each species' Gibbs
+    term and w = C exp(g) are computed at its first use, its omega-dot is stored after its
+    last reaction, and the reactions are ordered to keep the live set small;
   * k = A * exp(beta lnT - Ta/T) (A kept exact instead of exp(ln A + ...));
   * Troe's 10^x is exp10(x).
 """
@@ -313,6 +315,26 @@ class _RateEmitter:
         """Third-body sums from C, then gl_k = g_k log2(e) and w_k = C_k exp(g_k) (FP64).
         Concentrations are dead afterwards. Returns the eff-set index."""
         t, N = self.t, self.N
+        sidx = self.emit_third_body(S, cvar)
+        used = sorted({k for r in range(self.R) for k in range(N)
+                       if self.nu_f[r, k] or (self.rev[r] and self.nu_r[r, k])})
+        S('// ---- Gibbs energies g_k (FP64): gl = g log2(e) for the rate exponents, w = C exp(g)')
+        tmid = t['T_mid']
+        for tm in sorted({float(tmid[k]) for k in used}):
+            S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
+        L2E = 1.4426950408889634
+        for k in used:
+            S(f'const double g{k} = hi{_tag(tmid[k])} ? ({self._gibbs(t["g_hi"][k])}) : ({self._gibbs(t["g_lo"][k])});')
+            S(f'const double gl{k} = g{k} * {lit(L2E)};')
+            S(f'const double w{k} = {cvar}{k} * exp(g{k});')
+        S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
+        S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
+        return sidx
+
+    def emit_third_body(self, S, cvar='c'):
+        """Cm, the efficiency-set sums eff<i> and explicit colliders M<k> (FP64) from the
+        concentrations. Returns the eff-set index."""
+        t, N = self.t, self.N
         S('// ---- third-body concentrations')
         S(f'const double Cm = ' + ' + '.join(f'{cvar}{k}' for k in range(N)) + ';')
         sets, sidx = self.efficiency_sets()
@@ -332,19 +354,6 @@ class _RateEmitter:
                             if t['collider'][r] >= 0 and not self.has_eff[r]})
         for k in colliders:
             S(f'const double M{k} = {cvar}{k};')
-        used = sorted({k for r in range(self.R) for k in range(N)
-                       if self.nu_f[r, k] or (self.rev[r] and self.nu_r[r, k])})
-        S('// ---- Gibbs energies g_k (FP64): gl = g log2(e) for the rate exponents, w = C exp(g)')
-        tmid = t['T_mid']
-        for tm in sorted({float(tmid[k]) for k in used}):
-            S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
-        L2E = 1.4426950408889634
-        for k in used:
-            S(f'const double g{k} = hi{_tag(tmid[k])} ? ({self._gibbs(t["g_hi"][k])}) : ({self._gibbs(t["g_lo"][k])});')
-            S(f'const double gl{k} = g{k} * {lit(L2E)};')
-            S(f'const double w{k} = {cvar}{k} * exp(g{k});')
-        S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
-        S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
         return sidx
 
     def _third_w(self, r, sidx):
@@ -609,6 +618,118 @@ def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64', arrh32=
     return S.text()
 
 
+def _emit_bk1_live(t, info, variant='fp64', arrh32=False, order=None):
+    """w-form BK1 (FP64 or mixed_valu) in a register-liveness order: reactions follow
+    liveness.order_reactions; g_k and w_k are computed at the species' first use; wd_k
+    starts at its first net coefficient and is stored (with its enthalpy term added to the
+    heat release sum) after its last one. The NASA-7 coefficients of both pieces sit in an
+    LDS table read at a per-lane offset (T > T_mid selects the piece without a branch)."""
+    from . import liveness
+    N = int(t['n_species'])
+    em = _RateEmitter(t, info)
+    mixed = variant == 'mixed_valu'
+    em.arrh32 = mixed and arrh32
+    kname = f'kx_bk1_{variant}'
+    G, Wm, D = liveness.use_masks(t)
+    if order is None:
+        order = liveness.order_reactions(t)
+    order = [int(r) for r in order]
+    assert sorted(order) == list(range(em.R))
+    # NASA-7 table: [piece: 0 = T > T_mid, 1 = T <= T_mid][species][16]: Gibbs in 0..6,
+    # enthalpy in 8..13
+    PT = np.zeros((2, N, 16))
+    for k in range(N):
+        PT[0, k, :7], PT[0, k, 8:14] = t['g_hi'][k], t['h_hi'][k]
+        PT[1, k, :7], PT[1, k, 8:14] = t['g_lo'][k], t['h_lo'][k]
+    S = _Src()
+    S(f'__device__ const double {kname}_poly[{PT.size}] = {{' +
+      ', '.join(_literal(v) for v in PT.reshape(-1)) + '};', 0)
+    if mixed:
+        S('__device__ __forceinline__ double kx_pow2v(double y)', 0)
+        S('{', 0)
+        S('const double n = __builtin_rint(y);')
+        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
+        S('return __builtin_ldexp((double)m, (int)n);')
+        S('}', 0)
+    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S(f'{kname}(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('            const double pressure_R, const double pressure_,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
+    S('{', 0)
+    S(POOL_MARK)
+    S(f'__shared__ double poly_s[{PT.size}];')
+    S(f'for (int i_ = threadIdx.x; i_ < {PT.size}; i_ += {BLOCK}) poly_s[i_] = {kname}_poly[i_];')
+    S('__syncthreads();')
+    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double T = Tref * state[id];')
+    S('const double rcpT = 1.0 / T;')
+    S('const double lnT = log(T);')
+    S('const double T2 = T * T;')
+    S('const double T3 = T * T * T;')
+    S('const double T4 = T * T * T * T;')
+    S('const double rcpT2 = rcpT * rcpT;')
+    S('const double omlnT = 1.0 - lnT;')
+    S(f'const double log2T = lnT * {lit(1.4426950408889634)};')
+    S('const float Tf = (float)T;')
+    S('const float rcpTf = (float)rcpT;')
+    S('const float log2Tf = (float)log2T;')
+    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
+    S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
+    rcpW = t['rcpW']
+    for k in range(N):
+        S(f'double c{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S('const double rcpMbar = ' + ' + '.join(f'c{k}' for k in range(N)) + ';')
+    S('const double rho = pressure_R * rcpT / rcpMbar;')
+    for k in range(N):
+        S(f'c{k} *= rho;')
+    sidx = em.emit_third_body(S, 'c')
+    tmid = t['T_mid']
+    S('// ---- NASA-7 piece per T_mid: a per-lane offset into the LDS table')
+    for tm in sorted({float(v) for v in tmid}):
+        S(f'const double* P{_tag(tm)} = poly_s + (T > {lit(tm)} ? 0 : {N * 16});')
+    S(f'const double C0 = {lit(float(t["one_atm_R"]))} * rcpT;')
+    S(f'const double rcpC0 = {lit(float(t["R_one_atm"]))} * T;')
+    S('double sum_h = 0.0;')
+    L2E = 1.4426950408889634
+    W = t['W']
+    remaining = D.sum(0).astype(int)
+    have_w, have_wd = set(), set()
+    for r in order:
+        for k in np.flatnonzero(Wm[r]):
+            k = int(k)
+            if k in have_w:
+                continue
+            have_w.add(k)
+            P, o = f'P{_tag(tmid[k])}', 16 * k
+            S(f'const double g{k} = {P}[{o}] * omlnT + {P}[{o + 1}] * T + {P}[{o + 2}] * T2 + '
+              f'{P}[{o + 3}] * T3 + {P}[{o + 4}] * T4 + {P}[{o + 5}] * rcpT - {P}[{o + 6}];')
+            S(f'const double gl{k} = g{k} * {lit(L2E)};')
+            S(f'const double w{k} = c{k} * exp(g{k});')
+        em.emit_reaction_w(S, r, sidx, pow2='kx_pow2v' if mixed else 'exp2', fp64_falloff=not mixed)
+        for k in np.flatnonzero(D[r]):
+            k = int(k)
+            nu = int(em.net[r, k])
+            term = f'q{r}' if nu == 1 else (f'-q{r}' if nu == -1 else f'{lit(nu)} * q{r}')
+            if k in have_wd:
+                S(f'wd{k} += {term};')
+            else:
+                have_wd.add(k)
+                S(f'double wd{k} = {term};')
+            remaining[k] -= 1
+            if remaining[k] == 0:
+                P, o = f'P{_tag(tmid[k])}', 16 * k + 8
+                S(f'rates[id + offsetT + {k} * offset] = {lit(W[k])} * wd{k};')
+                S(f'sum_h += wd{k} * ({P}[{o}] + {P}[{o + 1}] * T + {P}[{o + 2}] * T2 + '
+                  f'{P}[{o + 3}] * T3 + {P}[{o + 4}] * T4 + {P}[{o + 5}] * rcpT);')
+    for k in range(N):
+        if k not in have_wd:
+            S(f'rates[id + offsetT + {k} * offset] = 0.0;')
+    S(f'rates[id] = {lit(-float(t["R"]))} * T * sum_h;')
+    S('}', 0)
+    return S.text()
+
+
 def _emit_h_RT_sum(S, t, active, wvar='w'):
     tmid = t['T_mid']
     for tm in sorted({float(tmid[k]) for k in active}):
@@ -791,7 +912,7 @@ def _with_pool(name, fn, *args, lds=False, block=BLOCK):
 
 
 DEFAULT_OPTS = {'bk1_sched_window': 0, 'bk1_wform': True, 'mixed': None, 'lds_pool': True,
-                'mixed_lds_pool': False, 'mixed_arrh32': True}
+                'mixed_lds_pool': False, 'mixed_arrh32': True, 'bk1_live': True}
 
 
 def generate(tables, info, opts=None):
@@ -810,7 +931,15 @@ def generate(tables, info, opts=None):
             parts.append(emit_rolled.emit_bk2(tables, info, True))
         parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
         return '\n\n'.join(parts) + '\n'
-    parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info, o['bk1_sched_window'], o['bk1_wform'], lds=o['lds_pool']))
+    order = None
+    if o['bk1_live']:
+        from . import liveness
+        order = liveness.order_reactions(tables)
+        parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_live, tables, info, 'fp64', False, order,
+                                lds=o['lds_pool']))
+    else:
+        parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info, o['bk1_sched_window'],
+                                o['bk1_wform'], lds=o['lds_pool']))
     if 'cond' in tables:
         parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False, lds=o['lds_pool']))
         parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, lds=o['lds_pool']))
@@ -818,8 +947,12 @@ def generate(tables, info, opts=None):
     from . import emit_mixed
     parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],
                             lds=o['mixed_lds_pool']))
-    parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_fp64, tables, info, 0, True, 'mixed_valu',
-                            o['mixed_arrh32'], lds=o['lds_pool']))
+    if o['bk1_live']:
+        parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_live, tables, info, 'mixed_valu',
+                                o['mixed_arrh32'], order, lds=o['lds_pool']))
+    else:
+        parts.append(_with_pool('kx_bk1_mixed_valu_k', _emit_bk1_fp64, tables, info, 0, True, 'mixed_valu',
+                                o['mixed_arrh32'], lds=o['lds_pool']))
     if 'cond' in tables:
         from . import emit_surrogate
         parts.append(_with_pool('kx_sur_k', emit_surrogate.emit, tables, info))
