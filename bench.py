@@ -218,12 +218,14 @@ def main():
         except KinetixError as e:          # form not generated for this mechanism
             variants[vname] = {'kernel': kname, 'absent': str(e)}
             continue
+        for _ in range(3):                  # back to full clocks after the CPU-only oracle phase
+            bk1(mode)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(5):
+        for _ in range(10):
             bk1(mode)
         e1.record(); torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 5
+        ms = e0.elapsed_time(e1) / 10
         r_ = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
         l2v = [float(np.linalg.norm(r_[1 + k] - ref[1 + k]) / np.linalg.norm(ref[1 + k]))
                for k in range(N) if np.linalg.norm(ref[1 + k]) > 0]
@@ -236,8 +238,10 @@ def main():
 
     # BK2 forms on the same inputs; the plain FP64 kernel is the judge of the surrogate over
     # every state
-    def timed(fn, reps=5):
-        fn(); torch.cuda.synchronize()
+    def timed(fn, reps=10):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
