@@ -25,7 +25,6 @@
 namespace {
 
 constexpr double kR = 1.380649e-23 * 6.02214076e23;   // kinetix.cpp:37
-constexpr int kBlock = 256;                            // launch bound of the generated kernels
 
 thread_local std::string g_err;
 
@@ -46,6 +45,7 @@ struct Mech {
   int device = 0;
   hipModule_t module = nullptr;
   std::map<std::string, hipFunction_t> fn;
+  std::map<std::string, unsigned> block;     // each kernel's launch bound (threads per block)
   int n_species = 0, n_active = 0, n_reactions = 0;
   std::vector<std::string> names;
   std::vector<double> W;
@@ -86,18 +86,19 @@ int read_meta(Mech* m, const std::string& path)
   return KX_OK;
 }
 
-// Launch helper: packs the kernel arguments, 1-D grid of kBlock-lane workgroups.
-int launch(Mech* m, const char* name, int n_states, void* stream, void** args,
-           unsigned block = kBlock, unsigned states_per_block = kBlock, unsigned shmem = 0)
+// Launch helper: packs the kernel arguments; 1-D grid, one state per thread, in workgroups
+// of the kernel's own launch bound (the generator picks it per kernel).
+int launch(Mech* m, const char* name, int n_states, void* stream, void** args)
 {
   auto it = m->fn.find(name);
   if (it == m->fn.end() || !it->second)
     return fail(KX_ERR_NO_KERNEL, std::string("kernel ") + name + " not in " + m->dir);
   if (n_states == 0) return KX_OK;
   KX_HIP(hipSetDevice(m->device));
-  const unsigned grid = (unsigned)((n_states + (long long)states_per_block - 1) / states_per_block);
-  KX_HIP(hipModuleLaunchKernel(it->second, grid, 1, 1, block, 1, 1, shmem,
-                               (hipStream_t)stream, args, nullptr));
+  const unsigned block = m->block.at(name);
+  const unsigned grid = (unsigned)((n_states + (long long)block - 1) / block);
+  KX_HIP(hipModuleLaunchKernel(it->second, grid, 1, 1, block, 1, 1, 0, (hipStream_t)stream,
+                               args, nullptr));
   return KX_OK;
 }
 
@@ -138,8 +139,20 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
                         "kx_sur_queue_init", "kx_sur_gather", "kx_sur_merge"}) {
     hipFunction_t f = nullptr;
-    if (hipModuleGetFunction(&f, m->module, k) == hipSuccess) m->fn[k] = f;
-    else (void)hipGetLastError();
+    if (hipModuleGetFunction(&f, m->module, k) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    int bound = 0;
+    if (hipFuncGetAttribute(&bound, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, f) != hipSuccess ||
+        bound <= 0 || bound > 1024 || bound % 64) {
+      (void)hipGetLastError();
+      hipModuleUnload(m->module);
+      delete m;
+      return fail(KX_ERR_HIP, std::string("no usable launch bound for ") + k);
+    }
+    m->fn[k] = f;
+    m->block[k] = (unsigned)bound;
   }
   std::lock_guard<std::mutex> lk(g_mu);
   *out = g_next++;

@@ -760,13 +760,18 @@ def _poly4(c):
             f'{lit(c[4])} * lnT4')
 
 
-def _emit_bk2_fp64(t, info, pstate=False):
+def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK):
     """pstate=False: kx_bk2_fp64, scalar p (the reference's transport kernel).
-    pstate=True: kx_bk2_fp64_pstate, p [Pa] per state, also writes D_km."""
+    pstate=True: kx_bk2_fp64_pstate, p [Pa] per state, also writes D_km.
+    block: threads per workgroup (the host launches with the kernel's launch bound). The
+    kernel's LDS constant pool (~85 KB for GRI-3.0) admits one workgroup per CU, so 512
+    threads (8 waves) is what gives 2 waves/SIMD: 8.4 ms instead of 14.5 at 256 threads for
+    4M states, despite ~160 VGPRs spilled to scratch; 768 and 1024 spill far more and are
+    2-3x slower."""
     N = int(t['n_species'])
     rcp_diff = bool(int(t.get('rcp_diff', 0)))
     S = _Src()
-    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
+    S(f'extern "C" __global__ void __launch_bounds__({block})', 0)
     if pstate:
         S('kx_bk2_fp64_pstate(const int n_states, const long long offsetT, const long long offset,', 0)
         S('            const double* __restrict__ p_state,', 0)
@@ -782,7 +787,7 @@ def _emit_bk2_fp64(t, info, pstate=False):
     S('            const int* __restrict__ n_dev)', 0)
     S('{', 0)
     S(POOL_MARK)
-    S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
+    S(f'const long long id = (long long)blockIdx.x * {block} + threadIdx.x;')
     S('// n_dev (optional): a state count known only on the device (audit/fallback queues)')
     S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
     S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
@@ -917,7 +922,7 @@ def _with_pool(name, fn, *args, lds=False, block=BLOCK):
 
 DEFAULT_OPTS = {'bk1_sched_window': 0, 'bk1_wform': True, 'mixed': None, 'lds_pool': True,
                 'mixed_lds_pool': False, 'mixed_arrh32': True, 'bk1_live': True,
-                'mixed_waves': 2}
+                'mixed_waves': 2, 'bk2_block': 512}
 
 
 def generate(tables, info, opts=None):
@@ -946,8 +951,10 @@ def generate(tables, info, opts=None):
         parts.append(_with_pool('kx_bk1_fp64_k', _emit_bk1_fp64, tables, info, o['bk1_sched_window'],
                                 o['bk1_wform'], lds=o['lds_pool']))
     if 'cond' in tables:
-        parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False, lds=o['lds_pool']))
-        parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, lds=o['lds_pool']))
+        parts.append(_with_pool('kx_bk2_fp64_k', _emit_bk2_fp64, tables, info, False, o['bk2_block'],
+                                lds=o['lds_pool'], block=o['bk2_block']))
+        parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, o['bk2_block'],
+                                lds=o['lds_pool'], block=o['bk2_block']))
     parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
     from . import emit_mixed
     parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],
