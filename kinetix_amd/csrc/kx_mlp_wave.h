@@ -74,21 +74,23 @@ __device__ __forceinline__ float gelu_fast(float x)
   return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-a));
 }
 
-// activations of the wave's 64 states for one layer: [k block][state block] fragments
-template <int KB>
-using Act = u32x4[KB][4];
+// activations of one pass (SB blocks of 16 states) for one layer: [k block][state block]
+// fragments. A pass of SB = 4 covers the wave's 64 states with 256 activation registers
+// (input + output layer); SB = 2 covers 32 states with 128, at twice the weight reads per state.
+template <int KB, int SB = 4>
+using Act = u32x4[KB][SB];
 
-// B fragments of the first layer from LDS: xs holds the wave's 64 states as rows of XS bf16
+// B fragments of the first layer from LDS: xs holds the pass's states as rows of XS bf16
 // (feature f at column f).
-template <int XS>
-__device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32>& in, int lane)
+template <int XS, int SB = 4>
+__device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32, SB>& in, int lane)
 {
   static_assert(XS % 4 == 0, "8-byte LDS reads");
   const int j = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int kb = 0; kb < kIn / 32; ++kb)
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb) {
+    for (int sb = 0; sb < SB; ++sb) {
       const uint16_t* r = xs + (16 * sb + j) * XS + 32 * kb + 4 * g;
       const uint2 lo = *(const uint2*)r;           // features kphi(kb, g, 0..3)
       const uint2 hi = *(const uint2*)(r + 16);    // features kphi(kb, g, 4..7)
@@ -100,22 +102,23 @@ __device__ __forceinline__ void load_input(const uint16_t* xs, Act<kIn / 32>& in
 // copies between them): while the MFMAs of block ob + 1 run, bias + GELU is applied to block
 // ob, so the MFMA pipe and the VALU overlap; the fragments and bias of block ob + 2 are
 // requested one block ahead (bias first: vmcnt retires loads in order).
-template <int KB>
-__device__ __forceinline__ void mfma_block(const u32x4 (&a)[KB], const f32x4& b, const Act<KB>& in,
-                                           f32x4 (&acc)[4])
+template <int KB, int SB>
+__device__ __forceinline__ void mfma_block(const u32x4 (&a)[KB], const f32x4& b,
+                                           const Act<KB, SB>& in, f32x4 (&acc)[SB])
 {
 #pragma unroll
-  for (int sb = 0; sb < 4; ++sb) acc[sb] = b;
+  for (int sb = 0; sb < SB; ++sb) acc[sb] = b;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb) acc[sb] = mfma(a[kb], in[kb][sb], acc[sb]);
+    for (int sb = 0; sb < SB; ++sb) acc[sb] = mfma(a[kb], in[kb][sb], acc[sb]);
 }
 
-__device__ __forceinline__ void gelu_block(const f32x4 (&acc)[4], Act<8>& out, int ob)
+template <int SB>
+__device__ __forceinline__ void gelu_block(const f32x4 (&acc)[SB], Act<8, SB>& out, int ob)
 {
 #pragma unroll
-  for (int sb = 0; sb < 4; ++sb) {
+  for (int sb = 0; sb < SB; ++sb) {
     const unsigned lo = pack2(gelu_fast(acc[sb][0]), gelu_fast(acc[sb][1]));
     const unsigned hi = pack2(gelu_fast(acc[sb][2]), gelu_fast(acc[sb][3]));
     out[ob >> 1][sb][(ob & 1) * 2] = lo;
@@ -132,14 +135,14 @@ __device__ __forceinline__ void load_block(const u32x4* __restrict__ wf, const f
   for (int kb = 0; kb < KB; ++kb) a[kb] = wf[(ob * KB + kb) * 64 + lane];
 }
 
-template <int KB>
+template <int KB, int SB>
 __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
-                                             const Act<KB>& in, Act<8>& out, int lane)
+                                             const Act<KB, SB>& in, Act<8, SB>& out, int lane)
 {
   constexpr int OB = kH / 16;
   u32x4 a0[KB], a1[KB];
   f32x4 b0, b1;
-  f32x4 accA[4], accB[4];
+  f32x4 accA[SB], accB[SB];
   load_block<KB>(wf, bias, 0, lane, a0, b0);
   load_block<KB>(wf, bias, 1, lane, a1, b1);
   mfma_block<KB>(a0, b0, in, accA);                       // block 0
@@ -157,8 +160,9 @@ __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const
 }
 
 // y[ob][sb][e]: output feature 16 ob + 4 (lane / 16) + e of state 16 sb + lane % 16
+template <int SB>
 __device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
-                                             const Act<8>& in, f32x4 (&y)[kOut / 16][4], int lane)
+                                             const Act<8, SB>& in, f32x4 (&y)[kOut / 16][SB], int lane)
 {
   const int g = lane >> 4;
   u32x4 a_cur[kH / 32], a_nxt[kH / 32];
@@ -174,27 +178,32 @@ __device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb) y[ob][sb] = b_cur;
+    for (int sb = 0; sb < SB; ++sb) y[ob][sb] = b_cur;
 #pragma unroll
     for (int kb = 0; kb < kH / 32; ++kb)
 #pragma unroll
-      for (int sb = 0; sb < 4; ++sb) y[ob][sb] = mfma(a_cur[kb], in[kb][sb], y[ob][sb]);
+      for (int sb = 0; sb < SB; ++sb) y[ob][sb] = mfma(a_cur[kb], in[kb][sb], y[ob][sb]);
 #pragma unroll
     for (int kb = 0; kb < kH / 32; ++kb) a_cur[kb] = a_nxt[kb];
     b_cur = b_nxt;
   }
 }
 
-// Full forward pass for the wave's 64 states. wf: fragment block (kFragTotal bf16, 16-B
-// aligned); p: FP32 parameter block (biases); xs: LDS rows of the standardised inputs.
-template <int XS>
+// Forward pass for 16 SB states (a wave's 64 at SB = 4). wf: fragment block (kFragTotal
+// bf16, 16-B aligned); p: FP32 parameter block (biases); xs: LDS rows of the standardised
+// inputs of the pass's first state onwards.
+template <int XS, int SB = 4>
 __device__ __forceinline__ void forward(const uint16_t* __restrict__ wfrag, const float* __restrict__ p,
-                                        const uint16_t* xs, f32x4 (&y)[kOut / 16][4], int lane)
+                                        const uint16_t* xs, f32x4 (&y)[kOut / 16][SB], int lane)
 {
+  // Opaque copies of the (uniform) parameter pointers: when forward() runs more than once per
+  // wave (passes over 32 states), the weight loads are otherwise loop-invariant and the
+  // compiler hoists or merges all 320 fragments (1,280 VGPRs) across the passes.
+  asm volatile("" : "+s"(wfrag), "+s"(p));
   const u32x4* wf = (const u32x4*)wfrag;
-  Act<kIn / 32> x0;
-  load_input<XS>(xs, x0, lane);
-  Act<8> h1, h2;
+  Act<kIn / 32, SB> x0;
+  load_input<XS, SB>(xs, x0, lane);
+  Act<8, SB> h1, h2;
   hidden_layer<kIn / 32>(wf + kF1 / 8, p + kB1, x0, h1, lane);
   hidden_layer<8>(wf + kF2 / 8, p + kB2, h1, h2, lane);
   hidden_layer<8>(wf + kF3 / 8, p + kB3, h2, h1, lane);

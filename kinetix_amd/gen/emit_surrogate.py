@@ -93,9 +93,10 @@ def emit(t, info):
         return (f'{float(c[0])!r}f + {float(c[1])!r}f * lT + {float(c[2])!r}f * lT2 + '
                 f'{float(c[3])!r}f * lT3 + {float(c[4])!r}f * lT4')
 
-    def state_model():
+    def state_model(diff=True):
         """Per-state code (one state per lane; T, p, id in scope): raw inputs raw[] and the
-        FP32 physics baselines bl[] = (ln mu, ln lambda, ln D_1m .. ln D_Nm)."""
+        FP32 physics baselines bl[] = (ln mu, ln lambda, ln D_1m .. ln D_Nm); with diff=False
+        only bl[0], bl[1] (the fused kernel recomputes the diffusion baselines per output)."""
         S('// FP32 throughout; logarithms and reciprocals on the hardware approximations')
         S('// (v_log_f32, v_rcp_f32: ~1 ulp), which the training features share')
         S(f'float raw[{N + 2}];')
@@ -136,14 +137,15 @@ def emit(t, info):
         S('float l1 = 0.0f, l2 = 0.0f;')
         for k in range(N):
             S(f'{{ const float l = {fpoly(t["cond"][k])}; l1 += x{k} * l; l2 += x{k} * __builtin_amdgcn_rcpf(l); }}')
-        S(f'float bl[{N + 2}];')
+        S(f'float bl[{N + 2 if diff else 2}];')
         S('bl[0] = kx_lnf(sqT * mu);')
         S('bl[1] = kx_lnf(sqT * 0.5f * (l1 + __builtin_amdgcn_rcpf(l2)));')
         S('const float lT15p = 1.5f * lT - lnp;')
-        for k in range(N):
-            d = t['diff'][k, pr[k]]
-            expr = f'-kx_lnf({fpoly(d)})' if rcp_diff else f'kx_lnf({fpoly(d)})'
-            S(f'bl[{k + 2}] = lT15p + {expr};')
+        if diff:
+            for k in range(N):
+                d = t['diff'][k, pr[k]]
+                expr = f'-kx_lnf({fpoly(d)})' if rcp_diff else f'kx_lnf({fpoly(d)})'
+                S(f'bl[{k + 2}] = lT15p + {expr};')
 
     def hull_and_inputs(store):
         """out = 1 if any raw input lies outside the hull box (or is NaN); store(i, expr)
@@ -231,12 +233,17 @@ def emit(t, info):
     S('}', 0)
 
     # ---------------- fused inference: features + MLP (BF16 MFMA) + outputs in one pass
-    BS = N + 2 + (1 - (N + 2) % 2)                 # odd LDS row stride of the baselines
-    S('// One state per lane for the physics part, 64 states per wavefront for the MLP')
-    S('// (kx_mlp_wave.h); LDS carries the inputs and baselines between the two layouts.')
-    S('// States flagged out of hull (or beyond n_states) get no output here: the caller')
-    S('// routes them to the plain kernel.')
-    S('extern "C" __global__ void __launch_bounds__(256, 1)', 0)
+    dtab = np.array([t['diff'][k, pr[k]] for k in range(N)], dtype=np.float32)      # (N, 5)
+    S(f'__device__ const float kx_sur_dpoly[{5 * N}] = {{' +
+      ', '.join(f'{float(v)!r}f' for v in dtab.reshape(-1)) + '};', 0)
+    S('// One state per lane for the physics part; the MLP (kx_mlp_wave.h) in two passes of 32', 0)
+    S('// states per wavefront, so activations take 128 VGPRs and two waves share a SIMD (one', 0)
+    S('// wave\'s MFMAs overlap the other\'s feature and GELU VALU work). LDS carries the inputs', 0)
+    S('// and four floats per state (ln mu and ln lambda baselines + output means, ln T,', 0)
+    S('// 1.5 ln T - ln p) between the two layouts; the diffusion baselines are recomputed per', 0)
+    S('// output from the FP32 fit table. States flagged out of hull (or beyond n_states) get', 0)
+    S('// no output here: the caller routes them to the plain kernel.', 0)
+    S('extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))', 0)
     S('kx_sur_infer(const int n_states, const long long offsetT, const long long offset,', 0)
     S('             const double* __restrict__ p_state, const double* __restrict__ state,', 0)
     S('             const float* __restrict__ norm, const float* __restrict__ ostat,', 0)
@@ -248,23 +255,24 @@ def emit(t, info):
     S('{', 0)
     S(E.POOL_MARK)
     S(f'__shared__ __attribute__((aligned(16))) uint16_t xs[4][64 * {XS}];')
-    S(f'__shared__ float bs[4][64 * {BS}];')
+    S('__shared__ __attribute__((aligned(16))) float ss[4][64 * 4];')
     S('__shared__ double rs[4][64];')
     S('__shared__ int fs[4][64];')
+    S(f'__shared__ float dps[{5 * N}];')
+    S(f'for (int i = threadIdx.x; i < {5 * N}; i += 256) dps[i] = kx_sur_dpoly[i];')
     S('const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;')
     S('const long long id = (long long)blockIdx.x * 256 + threadIdx.x;')
     S(f'uint16_t* xr = xs[wv] + lane * {XS};')
-    S(f'float* br = bs[wv] + lane * {BS};')
     S('if (id < n_states) {')
     S('const double T = Tref * state[id];')
     S('const double p = p_state[id];')
-    state_model()
+    state_model(diff=False)
     S(f'float xn[{IN}];')
     hull_and_inputs(lambda i, e: f'xn[{i}] = {e};')
     S(f'for (int i = {N + 2}; i < {IN}; ++i) xn[i] = 0.0f;')
     S(f'for (int i = 0; i < {IN}; i += 2)')
     S('  *(unsigned*)(xr + i) = kxmlp::pack2(xn[i], xn[i + 1]);')
-    S(f'for (int i = 0; i < {N + 2}; ++i) br[i] = bl[i] + ostat[i];')
+    S('*(kxmlp::f32x4*)(ss[wv] + 4 * lane) = (kxmlp::f32x4){bl[0] + ostat[0], bl[1] + ostat[1], lT, lT15p};')
     S(f'rs[wv][lane] = p * {lit(1.0 / float(t["R"]))} / (T * (double)ysum);')
     S('fs[wv][lane] = out;')
     S('flags[id] = out;')
@@ -275,22 +283,34 @@ def emit(t, info):
     S('fs[wv][lane] = 1;')
     S('}')
     S('__syncthreads();')
-    S('kxmlp::f32x4 y[kxmlp::kOut / 16][4];')
-    S(f'kxmlp::forward<{XS}>(wfrag, params, xs[wv], y, lane);')
     S('const int j = lane & 15, g = lane >> 4;')
+    S('#pragma unroll 1')
+    S('for (int h = 0; h < 2; ++h) {')
+    S('kxmlp::f32x4 y[kxmlp::kOut / 16][2];')
+    S(f'kxmlp::forward<{XS}, 2>(wfrag, params, xs[wv] + 32 * h * {XS}, y, lane);')
     S('#pragma unroll')
-    S('for (int sb = 0; sb < 4; ++sb) {')
-    S('  const int s = 16 * sb + j;')
+    S('for (int sb = 0; sb < 2; ++sb) {')
+    S('  const int s = 32 * h + 16 * sb + j;')
     S('  const long long sid = (long long)blockIdx.x * 256 + wv * 64 + s;')
     S('  if (fs[wv][s]) continue;')
     S('  const double rho = rs[wv][s];')
+    S('  const kxmlp::f32x4 q = *(const kxmlp::f32x4*)(ss[wv] + 4 * s);')
+    S('  const float lT = q[2], lT15p = q[3], lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT2 * lT2;')
     S('  #pragma unroll')
     S('  for (int ob = 0; ob < kxmlp::kOut / 16; ++ob)')
     S('  #pragma unroll')
     S('  for (int e = 0; e < 4; ++e) {')
     S('    const int f = 16 * ob + 4 * g + e;')
     S(f'    if (f >= {N + 2}) continue;')
-    S(f'    const float v = bs[wv][s * {BS} + f] + ostat[{IN} + f] * y[ob][sb][e];')
+    S('    float b;')
+    S('    if (f < 2) {')
+    S('      b = f == 0 ? q[0] : q[1];')
+    S('    } else {')
+    S('      const float* c = dps + 5 * (f - 2);')
+    S('      const float d = c[0] + c[1] * lT + c[2] * lT2 + c[3] * lT3 + c[4] * lT4;')
+    S(f'      b = lT15p {"-" if rcp_diff else "+"} kx_lnf(d) + ostat[f];')
+    S('    }')
+    S(f'    const float v = b + ostat[{IN} + f] * y[ob][sb][e];')
     S('    const double D = (double)__builtin_amdgcn_exp2f(v * 1.4426950408889634f);')
     S('    if (f == 0) viscosity[sid] = D;')
     S('    else if (f == 1) conductivity[sid] = D;')
@@ -299,6 +319,7 @@ def emit(t, info):
     S('      if (Dkm) Dkm[(f - 2) * offset + sid] = D;')
     S('    }')
     S('  }')
+    S('}')
     S('}')
     S('}', 0)
 
