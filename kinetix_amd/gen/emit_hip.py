@@ -375,6 +375,43 @@ class _RateEmitter:
             parts = ['0.0']
         return ' + '.join(parts) + extra
 
+    def plog_points(self, r):
+        """[(P [Pa], [(A, beta, Ta), ...]), ...] of PLOG reaction r, ascending in P."""
+        t = self.t
+        o0, o1 = int(t['plog_off'][r]), int(t['plog_off'][r + 1])
+        pts = []
+        for i in range(o0, o1):
+            k0, nk = int(t['plog_koff'][i]), int(t['plog_nk'][i])
+            rates = [(float(t['plog_A'][j]), float(t['plog_k'][j][1]), float(t['plog_k'][j][2]))
+                     for j in range(k0, k0 + nk)]
+            if any(A <= 0.0 for A, _, _ in rates):
+                raise NotImplementedError(f'reaction {r + 1}: PLOG rates with A <= 0')
+            pts.append((float(t['plog_P'][i]), rates))
+        if any(b[0] <= a[0] for a, b in zip(pts, pts[1:])):
+            raise ValueError(f'reaction {r + 1}: PLOG pressures not strictly ascending')
+        return pts
+
+    def _plog_point_log2k(self, rates):
+        """FP64 expression of log2 of the sum of the point's Arrhenius rates."""
+        terms = [self._log2_rate(A, b, Ta) for A, b, Ta in rates]
+        if len(terms) == 1:
+            return f'({terms[0]})'
+        return 'log2(' + ' + '.join(f'exp2({x})' for x in terms) + ')'
+
+    def _plog_log2k(self, r):
+        """FP64 expression of log2 k_f of PLOG reaction r at the pressure P (lnP = ln P)."""
+        pts = self.plog_points(r)
+        L = [self._plog_point_log2k(rates) for _, rates in pts]
+        if len(pts) == 1:
+            return L[0]
+        expr = L[-1]                                               # P > last point
+        for i in range(len(pts) - 2, -1, -1):
+            p1, p2 = pts[i][0], pts[i + 1][0]
+            rcp = 1.0 / (math.log(p2) - math.log(p1))
+            seg = (f'{L[i]} + ({L[i + 1]} - {L[i]}) * ((lnP - {lit(math.log(p1))}) * {lit(rcp)})')
+            expr = f'(P <= {lit(p2)} ? {seg} : {expr})'
+        return f'(P <= {lit(pts[0][0])} ? {L[0]} : {expr})'
+
     def _log2_rate32(self, A, beta, Ta):
         """FP32 expression of log2(A T^beta exp(-Ta/T)) with inline literal constants."""
         L2E = 1.4426950408889634
@@ -395,8 +432,12 @@ class _RateEmitter:
         S(f'double q{r};')
         S(f'{{ // {r + 1}: {self.equations[r] if self.equations else ""}')
         if kind == M.PLOG:
-            raise NotImplementedError('PLOG kernels are not emitted yet')
-        if getattr(self, 'arrh32', False) and not fp64_falloff:
+            # log2 k_f interpolated linearly in ln P between the bracketing pressure points
+            # and clamped outside them (reaction_rates.py:1064-1175); P is the launch's
+            # scalar pressure, so every branch is uniform. FP64 in every variant.
+            S(f'  const double ylk = {self._plog_log2k(r)};')
+            S(f'  double F = {pow2}(ylk{gsum});')
+        elif getattr(self, 'arrh32', False) and not fp64_falloff:
             # Arrhenius exponent in FP32 (inline literals); the Gibbs sum stays FP64
             S(f'  const float ya = {self._log2_rate32(t["A"][r], t["beta"][r], t["Ta"][r])};')
             S(f'  double F = {pow2}((double)ya{gsum});')
@@ -576,7 +617,8 @@ def _emit_bk1_fp64(t, info, sched_window=1, wform=False, variant='fp64', arrh32=
     S('const float Tf = (float)T;')
     S('const float rcpTf = (float)rcpT;')
     S('const float log2Tf = (float)log2T;')
-    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
+    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
+    S('(void)P; (void)lnP; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
     S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
     rcpW = t['rcpW']
     for k in range(N):
@@ -678,7 +720,8 @@ def _emit_bk1_live(t, info, variant='fp64', arrh32=False, order=None, waves=0):
     S('const float Tf = (float)T;')
     S('const float rcpTf = (float)rcpT;')
     S('const float log2Tf = (float)log2T;')
-    S('(void)pressure_; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
+    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
+    S('(void)P; (void)lnP; (void)log2T; (void)rcpT2; (void)Tf; (void)rcpTf; (void)log2Tf;')
     S('// ---- Y -> concentrations (negative mass fractions clipped to 0)')
     rcpW = t['rcpW']
     for k in range(N):

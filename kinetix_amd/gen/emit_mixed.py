@@ -147,7 +147,8 @@ def emit(t, info, opts=None):
     S('const double rcpT2 = rcpT * rcpT;')
     S('const float Tf = (float)T;')
     S('const float rcpTf = (float)rcpT;')
-    S('(void)pressure_; (void)rcpTf; (void)Tf; (void)rcpT2; (void)T2;')
+    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
+    S('(void)P; (void)lnP; (void)rcpTf; (void)Tf; (void)rcpT2; (void)T2;')
     rcpW = t['rcpW']
     for k in range(N):
         S(f'double c{k} = live ? fmax(0.0, state[id + offsetT + {k} * offset]) * {E.lit(rcpW[k])} : 0.0;')

@@ -239,12 +239,30 @@ def _bk1_tables(t, pre):
             row[6:10] = [float(v) for v in np.asarray(t['troe']).reshape(R, 4)[r]]
         if kind[r] == M.SRI:
             row[6:11] = [float(v) for v in np.asarray(t['sri']).reshape(R, 5)[r]]
-        if kind[r] == M.PLOG:
-            raise NotImplementedError('PLOG kernels are not emitted yet')
         dbl[r] = row
     # FP32 copies of the Arrhenius and low-pressure exponent coefficients (mixed variant)
     flt = dbl[:, :6].astype(np.float32)
-    out = [_table(f'{pre}_flt', flt, 'float'),_table(f'{pre}_fsp', f_sp, 'int'), _table(f'{pre}_rsp', r_sp, 'int'),
+    # PLOG points: per reaction a range of points (ascending P), per point a range of
+    # Arrhenius rates (log2 A, beta, Ta log2 e) summed at that pressure
+    em = E._RateEmitter(t, None)
+    plo, plP, pllnP, plrcp, plko, plnk, plk = [0], [], [], [], [], [], []
+    for r in range(R):
+        if kind[r] == M.PLOG:
+            pts = em.plog_points(r)
+            for i, (P, rates) in enumerate(pts):
+                plP.append(P)
+                pllnP.append(math.log(P))
+                plrcp.append(1.0 / (math.log(pts[i + 1][0]) - math.log(P)) if i + 1 < len(pts) else 0.0)
+                plko.append(len(plk) // 3)
+                plnk.append(len(rates))
+                for A, b, Ta in rates:
+                    plk += [math.log2(A), b, Ta * L2E]
+        plo.append(len(plP))
+    out = [_table(f'{pre}_flt', flt, 'float'),
+           _table(f'{pre}_plo', np.array(plo), 'int'), _table(f'{pre}_plP', np.array(plP)),
+           _table(f'{pre}_pllnP', np.array(pllnP)), _table(f'{pre}_plrcp', np.array(plrcp)),
+           _table(f'{pre}_plko', np.array(plko), 'int'), _table(f'{pre}_plnk', np.array(plnk), 'int'),
+           _table(f'{pre}_plk', np.array(plk)),_table(f'{pre}_fsp', f_sp, 'int'), _table(f'{pre}_rsp', r_sp, 'int'),
            _table(f'{pre}_nsp', n_sp, 'int'), _table(f'{pre}_nnu', n_nu),
            _table(f'{pre}_meta', meta, 'int'), _table(f'{pre}_dbl', dbl),
            _table(f'{pre}_setoff', np.array(set_off), 'int'), _table(f'{pre}_setsp', np.array(set_sp), 'int'),
@@ -277,6 +295,28 @@ def emit_bk1(t, info, variant='fp64'):
     else:
         S('return exp2(y);')
     S('}', 0)
+    S('// log2 of the summed Arrhenius rates of PLOG point i (FP64)', 0)
+    S(f'__device__ double {pre}_plog_point(int i, double log2T, double rcpT)', 0)
+    S('{', 0)
+    S(f'const double* c = {pre}_plk + 3 * {pre}_plko[i];')
+    S(f'const int nk = {pre}_plnk[i];')
+    S('if (nk == 1) return c[0] + c[1] * log2T - c[2] * rcpT;')
+    S('double s = 0.0;')
+    S('for (int j = 0; j < nk; ++j) s += exp2(c[3 * j] + c[3 * j + 1] * log2T - c[3 * j + 2] * rcpT);')
+    S('return log2(s);')
+    S('}', 0)
+    S('// log2 k_f of PLOG reaction r: linear in ln P between the bracketing points, clamped', 0)
+    S('// outside them (reaction_rates.py:1064-1175); P is uniform over the launch', 0)
+    S(f'__device__ double {pre}_plog(int r, double P, double lnP, double log2T, double rcpT)', 0)
+    S('{', 0)
+    S(f'const int o0 = {pre}_plo[r], o1 = {pre}_plo[r + 1];')
+    S(f'if (P <= {pre}_plP[o0]) return {pre}_plog_point(o0, log2T, rcpT);')
+    S(f'if (P > {pre}_plP[o1 - 1]) return {pre}_plog_point(o1 - 1, log2T, rcpT);')
+    S('int i = o0;')
+    S(f'while (i + 2 < o1 && P > {pre}_plP[i + 1]) ++i;')
+    S(f'const double L1 = {pre}_plog_point(i, log2T, rcpT), L2 = {pre}_plog_point(i + 1, log2T, rcpT);')
+    S(f'return L1 + (L2 - L1) * ((lnP - {pre}_pllnP[i]) * {pre}_plrcp[i]);')
+    S('}', 0)
     S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
     S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
     S('            const double pressure_R, const double pressure_,', 0)
@@ -284,7 +324,8 @@ def emit_bk1(t, info, variant='fp64'):
     S('{', 0)
     S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
-    S('(void)pressure_;')
+    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
+    S('(void)P; (void)lnP;')
     S('const double T = Tref * state[id];')
     S('const double rcpT = 1.0 / T;')
     S('const double lnT = log(T);')
@@ -326,14 +367,15 @@ def emit_bk1(t, info, variant='fp64'):
     S('  const int kind = mt[0], nf = mt[2], nr = mt[3];')
     if mixed:
         S(f'  const float* fp = {pre}_flt + 6 * r;')
-        S('  double ex = (double)(fp[0] + fp[1] * log2Tf - fp[2] * rcpTf);   // Arrhenius exponent in FP32')
+        S(f'  double ex = kind == {M.PLOG} ? {pre}_plog(r, P, lnP, log2T, rcpT)')
+        S('                              : (double)(fp[0] + fp[1] * log2Tf - fp[2] * rcpTf);   // FP32 Arrhenius exponent')
     else:
-        S('  double ex = dp[0] + dp[1] * log2T - dp[2] * rcpT;')
+        S(f'  double ex = kind == {M.PLOG} ? {pre}_plog(r, P, lnP, log2T, rcpT) : dp[0] + dp[1] * log2T - dp[2] * rcpT;')
     S('  double fwd = 1.0, rev = 1.0;')
     S(f'  for (int i = 0; i < nf; ++i) {{ const int k = {pre}_fsp[{MX} * r + i]; ex -= gl[k]; fwd *= w[k]; }}')
     S(f'  for (int i = 0; i < nr; ++i) rev *= w[{pre}_rsp[{MX} * r + i]];')
     S(f'  double F = {pre}_pow2(ex);')
-    S('  if (kind != 0) {')
+    S(f'  if (kind != 0 && kind != {M.PLOG}) {{')
     S('    const double M = mt[6] == 3 ? Ms[mt[7]] : (mt[6] == 2 ? Ccol[mt[7]] : Cm);')
     S(f'    if (kind == {M.THREE_BODY}) {{')
     S('      F *= M;')
