@@ -1,7 +1,9 @@
 """The reference's other mechanisms on the GPU, through the C ABI, against the oracle:
 LiDryer (9 species), NH3Konnov_edit (36, 3 PLOG reactions), chempolimi_edit (31, 21 PLOG),
-both straight-line, NH3Konnov_edit in the loop form too, and EtOHKonnov (129 species, 1,231
-reactions, loop form). BK1 FP64 <= 1e-12 of the reaction flux each output sums (forward
+both straight-line, NH3Konnov_edit in the loop form too, EtOHKonnov (129 species, 1,231
+reactions, loop form), and the seeded 40-species coverage mechanism (kinetix_amd/synth.py
+--coverage: every reaction form, incl. SRI, Lindemann, explicit colliders, irreversible
+reactions and PLOG points with summed rates) in both forms. BK1 FP64 <= 1e-12 of the reaction flux each output sums (forward
 and reverse separately) at pressures below, on, between and above the PLOG points; BK1 mixed <= 1e-5 relative L2 per species; BK2 and the
 thermodynamic kernel <= 1e-12; and the reference's known-answer states end to end."""
 import os
@@ -19,7 +21,8 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 P_ATM = 101325.0
-MECHS = ['LiDryer', 'NH3Konnov_edit', 'NH3Konnov_edit_rolled', 'chempolimi_edit', 'EtOHKonnov']
+MECHS = ['LiDryer', 'NH3Konnov_edit', 'NH3Konnov_edit_rolled', 'chempolimi_edit', 'EtOHKonnov',
+         'synth40cov', 'synth40cov_rolled']
 # below, on and between NH3Konnov's points (0.1, 1, 10 atm); above both mechanisms' last point
 PRESSURES_ATM = (0.05, 1.0, 3.0, 20.0, 2000.0)
 
@@ -139,6 +142,8 @@ def test_known_answer_states(setup):
     R T sum_k flux_k |h_k / RT|)."""
     name, info, t, kx, orc = setup
     base = name.replace('_rolled', '')
+    if base.startswith('synth'):
+        pytest.skip('synthetic mechanism: no known-answer records')
     na = info['n_active_species']
     for state in ('initial', 'ignition', 'final'):
         d = kat.load(base, state)
