@@ -20,6 +20,8 @@
 #ifndef KINETIX_MI_H
 #define KINETIX_MI_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -47,6 +49,17 @@ const char *kx_last_error(void);
  * this build generates and compiles ahead of time, so init only loads the code object.
  */
 int kx_init(const char *mech_dir, int device_id, kx_mech_t *out);
+
+/*
+ * The host-application hook of kinetix::init (kinetixBuildKernel_t, kinetix.hpp:11-13;
+ * kinetix.cpp:499-502, 542): the reference lets a host code supply the kernel build; here
+ * the application supplies the built artefacts from memory instead of a directory -- the
+ * text of mech.txt (mech_txt_bytes bytes, not NUL-terminated necessarily) and the gfx950
+ * code object (as written to kernels.hsaco; it must stay valid for the call only). For
+ * codes that embed the mechanism in their binary or run the generator in their own build.
+ */
+int kx_init_image(const char *mech_txt, size_t mech_txt_bytes, const void *code_object,
+                  size_t code_object_bytes, int device_id, kx_mech_t *out);
 
 /* kinetix::build (kinetix.cpp:609-784): reference pressure [Pa], temperature [K] and mass
  * fractions (n_species values); the state arrays hold T/T_ref and p arguments are p/p_ref. */

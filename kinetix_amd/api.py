@@ -41,6 +41,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.kx_last_error.restype = C.c_char_p
         L.kx_init.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+        L.kx_init_image.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t, C.c_int,
+                                    C.POINTER(C.c_int)]
         L.kx_build.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
         L.kx_finalize.argtypes = [C.c_int]
         for fn in ('kx_production_rates', 'kx_production_rates_mixed',
@@ -108,10 +110,17 @@ def mech_dir(name):
 class Kinetix:
     """One mechanism loaded on one device (the reference's process-global kinetix:: state)."""
 
-    def __init__(self, mech='gri30', device=0):
-        self.dir = mech_dir(mech)
+    def __init__(self, mech='gri30', device=0, image=None):
+        """image: optional (mech_txt bytes, code object bytes) loaded from memory through
+        kx_init_image (the host-application hook) instead of the mechanism directory."""
         h = C.c_int(0)
-        _check(lib().kx_init(self.dir.encode(), int(device), C.byref(h)))
+        if image is None:
+            self.dir = mech_dir(mech)
+            _check(lib().kx_init(self.dir.encode(), int(device), C.byref(h)))
+        else:
+            self.dir = None
+            meta, code = image
+            _check(lib().kx_init_image(meta, len(meta), code, len(code), int(device), C.byref(h)))
         self.h = h.value
         self.device = device
         self.n_species = _check(lib().kx_n_species(self.h))

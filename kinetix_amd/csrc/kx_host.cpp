@@ -66,10 +66,10 @@ Mech* get(kx_mech_t h)
   return it == g_mechs.end() ? nullptr : it->second;
 }
 
-int read_meta(Mech* m, const std::string& path)
+// mech.txt: one "key values..." line each for n_species, n_active_species, n_reactions,
+// species (names) and molar_mass (kg/mol), in any order
+int parse_meta(Mech* m, std::istream& f, const std::string& what)
 {
-  std::ifstream f(path);
-  if (!f) return fail(KX_ERR_IO, "cannot open " + path);
   std::string line;
   while (std::getline(f, line)) {
     std::istringstream is(line);
@@ -82,8 +82,15 @@ int read_meta(Mech* m, const std::string& path)
     else if (key == "molar_mass") { double w; while (is >> w) m->W.push_back(w); }
   }
   if (m->n_species <= 0 || (int)m->names.size() != m->n_species || (int)m->W.size() != m->n_species)
-    return fail(KX_ERR_IO, "malformed " + path);
+    return fail(KX_ERR_IO, "malformed " + what);
   return KX_OK;
+}
+
+int read_meta(Mech* m, const std::string& path)
+{
+  std::ifstream f(path);
+  if (!f) return fail(KX_ERR_IO, "cannot open " + path);
+  return parse_meta(m, f, path);
 }
 
 // Launch helper: packs the kernel arguments; 1-D grid, one state per thread, in workgroups
@@ -119,21 +126,10 @@ extern "C" {
 
 const char* kx_last_error(void) { return g_err.c_str(); }
 
-int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
+// Registers the module's kernels with their launch bounds and publishes the handle; takes
+// ownership of m (deleted on failure).
+static int finish_init(Mech* m, kx_mech_t* out)
 {
-  if (!mech_dir || !out) return fail(KX_ERR_INVALID_ARG, "null argument");
-  Mech* m = new Mech();
-  m->dir = mech_dir;
-  m->device = device_id;
-  int rc = read_meta(m, m->dir + "/mech.txt");
-  if (rc) { delete m; return rc; }
-  hipError_t e = hipSetDevice(device_id);
-  if (e == hipSuccess) e = hipModuleLoad(&m->module, (m->dir + "/kernels.hsaco").c_str());
-  if (e != hipSuccess) {
-    delete m;
-    return fail(e == hipErrorFileNotFound ? KX_ERR_IO : KX_ERR_HIP,
-                std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
-  }
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
@@ -158,6 +154,44 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
   *out = g_next++;
   g_mechs[*out] = m;
   return KX_OK;
+}
+
+int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
+{
+  if (!mech_dir || !out) return fail(KX_ERR_INVALID_ARG, "null argument");
+  Mech* m = new Mech();
+  m->dir = mech_dir;
+  m->device = device_id;
+  int rc = read_meta(m, m->dir + "/mech.txt");
+  if (rc) { delete m; return rc; }
+  hipError_t e = hipSetDevice(device_id);
+  if (e == hipSuccess) e = hipModuleLoad(&m->module, (m->dir + "/kernels.hsaco").c_str());
+  if (e != hipSuccess) {
+    delete m;
+    return fail(e == hipErrorFileNotFound ? KX_ERR_IO : KX_ERR_HIP,
+                std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
+  }
+  return finish_init(m, out);
+}
+
+int kx_init_image(const char* mech_txt, size_t mech_txt_bytes, const void* code_object,
+                  size_t code_object_bytes, int device_id, kx_mech_t* out)
+{
+  if (!mech_txt || !code_object || !out || code_object_bytes == 0)
+    return fail(KX_ERR_INVALID_ARG, "null argument");
+  Mech* m = new Mech();
+  m->dir = "<image>";
+  m->device = device_id;
+  std::istringstream meta(std::string(mech_txt, mech_txt_bytes));
+  int rc = parse_meta(m, meta, "mechanism image text");
+  if (rc) { delete m; return rc; }
+  hipError_t e = hipSetDevice(device_id);
+  if (e == hipSuccess) e = hipModuleLoadData(&m->module, code_object);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(KX_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+  }
+  return finish_init(m, out);
 }
 
 int kx_build(kx_mech_t h, double ref_pressure, double ref_temperature, const double* refY)

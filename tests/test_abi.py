@@ -59,3 +59,13 @@ def test_unknown_handle_is_an_error_not_a_launch(L):
     assert rc == -1                                                    # KX_ERR_INVALID_ARG
     rc = L.kx_mlp_forward(C.c_void_p(16), 65, C.c_void_p(16), C.c_void_p(16), None)
     assert rc == -1                                                    # batch not a multiple of 64
+
+
+def test_init_image_rejects_malformed_text_before_any_device_call(L):
+    """kx_init_image parses the mechanism text before it touches the device."""
+    h = C.c_int(0)
+    meta = b'n_species 2\nspecies A\nmolar_mass 0.002 0.032\n'          # one name for two species
+    rc = L.kx_init_image(meta, len(meta), b'\x7fELF', 4, 0, C.byref(h))
+    assert rc == -5                                                    # KX_ERR_IO
+    assert b'malformed mechanism image text' in L.kx_last_error()
+    assert L.kx_init_image(None, 0, b'\x7fELF', 4, 0, C.byref(h)) == -1
