@@ -4,7 +4,7 @@
 
 One step = one BK1 launch + one BK2 launch over all N_STATES states, inputs resident in HBM
 (the reference's timing loop, benchmark/src/bk.cpp:694-776, without H2D/D2H). Prints ONE
-JSON line (driver contract) with the roofline of the dominant kernel, the CPU baseline
+JSON line with the metric, the roofline of the dominant kernel, the CPU baseline
 (the reference's own FP64 routines, oracle/_ref, SERIAL flags, timed on this host's cores on
 a bounded sample), and the error of every output against the FP64 oracle.
 
@@ -236,8 +236,8 @@ def main():
             v['speedup_vs_fp64'] = round(variants['fp64']['ms'] / v['ms'], 3)
     bk1(); torch.cuda.synchronize()               # leave the headline variant's output
 
-    # BK2 forms on the same inputs; the plain FP64 kernel is the judge of the surrogate over
-    # every state
+    # BK2 forms on the same inputs; the surrogate's error is measured against the plain FP64
+    # kernel over every state
     def timed(fn, reps=10):
         for _ in range(3):
             fn()

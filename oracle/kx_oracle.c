@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY: the CPU oracle. Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load this library, and only as the checker (or the timed CPU
- * baseline); the product path never calls it.
+ * bench.py's cpu_baseline leg load this library, and only to check GPU results against (or as
+ * the timed CPU baseline); the product path never calls it.
  *
  * Plain-C, data-driven restatement of the reference's FP64 hot path for ANY mechanism,
  * reading the dense tables the generator writes (kinetix_amd/gen/tables.py):

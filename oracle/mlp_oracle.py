@@ -1,5 +1,5 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy restatement of the surrogate MLP (forward, backward of
-the masked MSE, one Adam step), the checker for kinetix_amd/csrc/kx_mlp.hip.
+the masked MSE, one Adam step), the reference the tests hold kinetix_amd/csrc/kx_mlp.hip to.
 
 Layer layout and parameter block as in kx_mlp.hip: W1[256][64] b1 W2[256][256] b2 W3 b3
 W4[64][256] b4; GELU in its tanh form; loss = mean over the batch and the first 55 outputs of

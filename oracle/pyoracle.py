@@ -2,8 +2,8 @@
 the reference's compiled GRI-3.0 routines (oracle/_ref).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module,
-and only as the checker or the timed CPU baseline. The product path (kinetix_amd) never
-does.
+and only to check GPU results against or as the timed CPU baseline. The product path
+(kinetix_amd) never does.
 """
 from __future__ import annotations
 
