@@ -99,6 +99,7 @@ def cpu_baseline(state, p, cores, target_s, mech='gri30'):
     run(s[:, :min(n, 1024)])                     # warm
     t0 = time.perf_counter(); run(s); dt = time.perf_counter() - t0
     return {'value': round(n / dt / 1e6, 4), 'unit': 'Mstates/s', 'cores': cores, 'kind': kind,
+            'per_core': round(n / dt / 1e6 / cores, 4),
             'sample': f'{n} of the {n_all} states, BK1 + BK2 FP64, '
                       f'{"reference GRI routines (SERIAL flags)" if kind == "reference" else "oracle port (-O3 -ffast-math)"}, '
                       f'{cores} threads, {dt:.1f} s'}
@@ -384,6 +385,7 @@ def main():
         'roofline': roofline,
         'kernels': kern,
         'bk1_Mstates_s': round(n / m1 / 1e3, 2), 'bk2_Mstates_s': round(n / m2 / 1e3, 2),
+        'bk1_median_ms': round(float(np.median(t_bk1)), 4), 'bk2_median_ms': round(float(np.median(t_bk2)), 4),
         'bk1_GRXN_s': round(n * kx.nReactions() / m1 / 1e6, 2),
         'bk2_GDOF_s': round(n * (N + 2) / m2 / 1e6, 2),
         'err_vs_fp64_oracle': err,
