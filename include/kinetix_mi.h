@@ -68,6 +68,22 @@ int kx_build(kx_mech_t m, double ref_pressure, double ref_temperature,
 
 int kx_finalize(kx_mech_t m);
 
+/*
+ * The generated routine files (mechs/<m>/routines: mech.h, rates.cpp, enthalpy_RT.cpp,
+ * heat_capacity_R.cpp, conductivity.cpp, viscosity.cpp, diffusivity.cpp -- the reference's
+ * device-routine ABI, reaction_rates.py:1271-1273, examples/KinetiX/kinetix_test.hpp) called
+ * from kernels structured as the reference's productionRates and transport kernels
+ * (benchmark/okl/productionRates.okl:10-62, transportProps.okl:11-47): the form a host code
+ * that includes those files runs. Same arguments and outputs as kx_production_rates and
+ * kx_mixture_avg_transport, FP64; slower than those fused kernels.
+ */
+int kx_production_rates_routines(kx_mech_t m, int n_states, int offsetT, int offset,
+                                 double pressure, const double *d_state, double *d_rates,
+                                 void *stream);
+int kx_mixture_avg_transport_routines(kx_mech_t m, int n_states, int offsetT, int offset,
+                                      double pressure, const double *d_state, double *d_viscosity,
+                                      double *d_conductivity, double *d_rhoD, void *stream);
+
 /* kinetix::productionRates (kinetix.cpp:786-813) -> benchmark/okl/productionRates.okl.
  * pressure = p/p_ref (scalar for all states). rates: (n_species+1) rows in the state
  * layout; rates[id] = heat release rate [W/m^3], rates[id+offsetT+k*offset] = W_k wdot_k

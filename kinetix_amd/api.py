@@ -46,12 +46,12 @@ def lib():
         L.kx_build.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
         L.kx_finalize.argtypes = [C.c_int]
         for fn in ('kx_production_rates', 'kx_production_rates_mixed',
-                   'kx_production_rates_mixed_valu'):
+                   'kx_production_rates_mixed_valu', 'kx_production_rates_routines'):
             getattr(L, fn).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
-        L.kx_mixture_avg_transport.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
-                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                               C.c_void_p]
+        for fn in ('kx_mixture_avg_transport', 'kx_mixture_avg_transport_routines'):
+            getattr(L, fn).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.kx_mixture_avg_transport_pstate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
                                                       C.c_void_p, C.c_void_p, C.c_void_p,
                                                       C.c_void_p, C.c_void_p, C.c_void_p,
@@ -151,13 +151,15 @@ class Kinetix:
     # kinetix::productionRates
     def productionRates(self, n_states, offsetT, offset, pressure, o_state, o_rates, mixed=False):
         """mixed: False (FP64), True or 'mfma' (mixed precision, q.nu on FP32 MFMA),
-        'valu' (mixed precision, q.nu on FP64 VALU)."""
+        'valu' (mixed precision, q.nu on FP64 VALU), 'routines' (FP64 through the generated
+        routine files, as the reference's kernel calls them)."""
         need = offsetT + (self.n_species - 1) * offset + n_states if n_states else 0
         _check_tensor('state', o_state, need)
         _check_tensor('rates', o_rates, need)
         fn = {False: lib().kx_production_rates, True: lib().kx_production_rates_mixed,
               'mfma': lib().kx_production_rates_mixed,
-              'valu': lib().kx_production_rates_mixed_valu}[mixed]
+              'valu': lib().kx_production_rates_mixed_valu,
+              'routines': lib().kx_production_rates_routines}[mixed]
         _check(fn(self.h, int(n_states), int(offsetT), int(offset), float(pressure), _ptr(o_state),
                   _ptr(o_rates), _stream(o_state)))
 
@@ -166,17 +168,19 @@ class Kinetix:
 
     # kinetix::mixtureAvgTransportProps
     def mixtureAvgTransportProps(self, n_states, offsetT, offset, pressure, o_state, o_viscosity,
-                                 o_thermalConductivity, o_densityDiffCoeffs):
+                                 o_thermalConductivity, o_densityDiffCoeffs, routines=False):
+        """routines=True: through the generated routine files, as the reference's transport
+        kernel calls them (kx_mixture_avg_transport_routines)."""
         need = offsetT + (self.n_species - 1) * offset + n_states if n_states else 0
         _check_tensor('state', o_state, need)
         _check_tensor('viscosity', o_viscosity, n_states)
         _check_tensor('thermalConductivity', o_thermalConductivity, n_states)
         _check_tensor('densityDiffCoeffs', o_densityDiffCoeffs,
                       (self.n_species - 1) * offset + n_states if n_states else 0)
-        _check(lib().kx_mixture_avg_transport(self.h, int(n_states), int(offsetT), int(offset),
-                                              float(pressure), _ptr(o_state), _ptr(o_viscosity),
-                                              _ptr(o_thermalConductivity), _ptr(o_densityDiffCoeffs),
-                                              _stream(o_state)))
+        fn = lib().kx_mixture_avg_transport_routines if routines else lib().kx_mixture_avg_transport
+        _check(fn(self.h, int(n_states), int(offsetT), int(offset), float(pressure), _ptr(o_state),
+                  _ptr(o_viscosity), _ptr(o_thermalConductivity), _ptr(o_densityDiffCoeffs),
+                  _stream(o_state)))
 
     def mixtureAvgTransportPropsPstate(self, n_states, offsetT, offset, o_p, o_state, o_viscosity,
                                        o_thermalConductivity, o_densityDiffCoeffs, o_Dkm=None):

@@ -34,7 +34,8 @@ def build_mechanism(mech_dir, extra=()):
     src = os.path.join(mech_dir, 'kernels.hip')
     out = os.path.join(mech_dir, 'kernels.hsaco')
     inc = '-I' + os.path.join(PKG, 'csrc')          # kx_mlp_wave.h (surrogate inference)
-    _run([HIPCC, '--genco', *COMMON, inc, *extra, src, '-o', out])
+    rinc = '-I' + os.path.join(mech_dir, 'routines')   # the generated routine files
+    _run([HIPCC, '--genco', *COMMON, inc, rinc, *extra, src, '-o', out])
     return out
 
 

@@ -133,7 +133,8 @@ static int finish_init(Mech* m, kx_mech_t* out)
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
-                        "kx_sur_queue_init", "kx_sur_gather", "kx_sur_merge"}) {
+                        "kx_sur_queue_init", "kx_sur_gather", "kx_sur_merge",
+                        "kx_bk1_routines", "kx_bk2_routines"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) != hipSuccess) {
       (void)hipGetLastError();
@@ -263,6 +264,28 @@ int kx_production_rates_mixed_valu(kx_mech_t h, int n, int offsetT, int offset, 
 {
   return production_rates_impl("kx_bk1_mixed_valu", h, n, offsetT, offset, pressure, d_state,
                                d_rates, stream);
+}
+
+int kx_production_rates_routines(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                                 const double* d_state, double* d_rates, void* stream)
+{
+  return production_rates_impl("kx_bk1_routines", h, n, offsetT, offset, pressure, d_state, d_rates,
+                               stream);
+}
+
+int kx_mixture_avg_transport_routines(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                                      const double* d_state, double* d_visc, double* d_cond,
+                                      double* d_rhoD, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_rhoD);
+  if (rc) return rc;
+  if (n > 0 && (!d_visc || !d_cond)) return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  const double* s = d_state;
+  void* args[] = {&n, &oT, &o, &pressure, &s, &d_cond, &d_visc, &d_rhoD, &Tref};
+  return launch(m, "kx_bk2_routines", n, stream, args);
 }
 
 static int transport_impl(kx_mech_t h, int n, int offsetT, int offset, double pressure,

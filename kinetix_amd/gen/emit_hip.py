@@ -1015,7 +1015,12 @@ def generate(tables, info, opts=None):
 
 
 def write(out_dir, mech, tables, opts=None):
+    """kernels.hip (every kernel, then the reference-structured kernels over the routine
+    files) and the routine files themselves (out_dir/routines, emit_routines)."""
+    from . import emit_routines
     from .tables import meta
-    src = generate(tables, meta(mech), opts)
+    info = meta(mech)
+    src = generate(tables, info, opts)
+    emit_routines.write(out_dir, tables, info)
     with open(os.path.join(out_dir, 'kernels.hip'), 'w') as fh:
-        fh.write(src)
+        fh.write(src + '\n' + emit_routines.kernels(tables, info))

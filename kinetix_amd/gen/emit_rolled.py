@@ -273,74 +273,45 @@ def _bk1_tables(t, pre):
     return out, len(sets), MX, MS
 
 
-def emit_bk1(t, info, variant='fp64'):
-    """Rolled BK1. variant 'fp64': FP64 exponentials and falloff; 'mixed_valu': Arrhenius
-    exponent in FP32, its sum with the FP64 Gibbs terms through kx_pow2 (FP64 rint, FP32
-    exp2 of the fraction), falloff in FP32; Gibbs terms, net rates and the wdot sums in
-    FP64 -- the straight-line forms' split."""
+def plog_helpers(pre):
+    """Device helpers of the loop-form PLOG evaluation (tables of prefix pre), as lines."""
+    H = E._Src()
+    H('// log2 of the summed Arrhenius rates of PLOG point i (FP64)', 0)
+    H(f'__device__ double {pre}_plog_point(int i, double log2T, double rcpT)', 0)
+    H('{', 0)
+    H(f'const double* c = {pre}_plk + 3 * {pre}_plko[i];')
+    H(f'const int nk = {pre}_plnk[i];')
+    H('if (nk == 1) return c[0] + c[1] * log2T - c[2] * rcpT;')
+    H('double s = 0.0;')
+    H('for (int j = 0; j < nk; ++j) s += exp2(c[3 * j] + c[3 * j + 1] * log2T - c[3 * j + 2] * rcpT);')
+    H('return log2(s);')
+    H('}', 0)
+    H('// log2 k_f of PLOG reaction r: linear in ln P between the bracketing points, clamped', 0)
+    H('// outside them (reaction_rates.py:1064-1175); P is uniform over the launch', 0)
+    H(f'__device__ double {pre}_plog(int r, double P, double lnP, double log2T, double rcpT)', 0)
+    H('{', 0)
+    H(f'const int o0 = {pre}_plo[r], o1 = {pre}_plo[r + 1];')
+    H(f'if (P <= {pre}_plP[o0]) return {pre}_plog_point(o0, log2T, rcpT);')
+    H(f'if (P > {pre}_plP[o1 - 1]) return {pre}_plog_point(o1 - 1, log2T, rcpT);')
+    H('int i = o0;')
+    H(f'while (i + 2 < o1 && P > {pre}_plP[i + 1]) ++i;')
+    H(f'const double L1 = {pre}_plog_point(i, log2T, rcpT), L2 = {pre}_plog_point(i + 1, log2T, rcpT);')
+    H(f'return L1 + (L2 - L1) * ((lnP - {pre}_pllnP[i]) * {pre}_plrcp[i]);')
+    H('}', 0)
+    return H.text().split('\n')
+
+
+def emit_bk1_core(S, t, pre, nsets, MX, MS, mixed):
+    """Loop-form production rates of one state, shared by the loop-form kernels and the
+    kinetix_species_rates device routine. In scope: T, rcpT, lnT, T2, T3, T4, log2T, Tf,
+    rcpTf, log2Tf, P, lnP and the private arrays w[N] (concentrations C_k on entry, w_k =
+    C_k exp(g_k) on exit) and wd[N] (omega-dot on exit). Tables and helpers of prefix pre
+    (_bk1_tables, the pow2 and PLOG helpers) precede it."""
     N, R = int(t['n_species']), int(t['n_reactions'])
-    name = f'kx_bk1_{variant}'
-    pre = f'{name}_r'
-    mixed = variant != 'fp64'
-    tabs, nsets, MX, MS = _bk1_tables(t, pre)
-    S = E._Src()
-    for line in tabs:
-        S(line, 0)
-    S(f'__device__ __forceinline__ double {pre}_pow2(double y)', 0)
-    S('{', 0)
-    if mixed:
-        S('const double n = __builtin_rint(y);')
-        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
-        S('return __builtin_ldexp((double)m, (int)n);')
-    else:
-        S('return exp2(y);')
-    S('}', 0)
-    S('// log2 of the summed Arrhenius rates of PLOG point i (FP64)', 0)
-    S(f'__device__ double {pre}_plog_point(int i, double log2T, double rcpT)', 0)
-    S('{', 0)
-    S(f'const double* c = {pre}_plk + 3 * {pre}_plko[i];')
-    S(f'const int nk = {pre}_plnk[i];')
-    S('if (nk == 1) return c[0] + c[1] * log2T - c[2] * rcpT;')
-    S('double s = 0.0;')
-    S('for (int j = 0; j < nk; ++j) s += exp2(c[3 * j] + c[3 * j + 1] * log2T - c[3 * j + 2] * rcpT);')
-    S('return log2(s);')
-    S('}', 0)
-    S('// log2 k_f of PLOG reaction r: linear in ln P between the bracketing points, clamped', 0)
-    S('// outside them (reaction_rates.py:1064-1175); P is uniform over the launch', 0)
-    S(f'__device__ double {pre}_plog(int r, double P, double lnP, double log2T, double rcpT)', 0)
-    S('{', 0)
-    S(f'const int o0 = {pre}_plo[r], o1 = {pre}_plo[r + 1];')
-    S(f'if (P <= {pre}_plP[o0]) return {pre}_plog_point(o0, log2T, rcpT);')
-    S(f'if (P > {pre}_plP[o1 - 1]) return {pre}_plog_point(o1 - 1, log2T, rcpT);')
-    S('int i = o0;')
-    S(f'while (i + 2 < o1 && P > {pre}_plP[i + 1]) ++i;')
-    S(f'const double L1 = {pre}_plog_point(i, log2T, rcpT), L2 = {pre}_plog_point(i + 1, log2T, rcpT);')
-    S(f'return L1 + (L2 - L1) * ((lnP - {pre}_pllnP[i]) * {pre}_plrcp[i]);')
-    S('}', 0)
-    S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
-    S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
-    S('            const double pressure_R, const double pressure_,', 0)
-    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
-    S('{', 0)
-    S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
-    S('if (id >= n_states) return;')
-    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
-    S('(void)P; (void)lnP;')
-    S('const double T = Tref * state[id];')
-    S('const double rcpT = 1.0 / T;')
-    S('const double lnT = log(T);')
-    S('const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;')
-    S('const double log2T = lnT * 1.4426950408889634;')
-    S('const float Tf = (float)T, rcpTf = (float)rcpT, log2Tf = (float)log2T;')
-    S('(void)Tf; (void)rcpTf; (void)log2Tf;')
-    S(f'double w[{N}], gl[{N}], wd[{N}], Ms[{max(nsets, 1)}];   // per-lane private arrays')
-    S('double rcpMbar = 0.0;')
-    S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) {{ const double c = fmax(0.0, state[id + offsetT + k * offset]) * {pre}_rcpW[k]; w[k] = c; rcpMbar += c; }}')
-    S('const double rho = pressure_R * rcpT / rcpMbar;')
+    S(f'double gl[{N}], Ms[{max(nsets, 1)}];')
     S('double Cm = 0.0;')
     S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) {{ w[k] *= rho; Cm += w[k]; wd[k] = 0.0; }}')
+    S(f'for (int k = 0; k < {N}; ++k) {{ Cm += w[k]; wd[k] = 0.0; }}')
     S('// third-body concentrations of the efficiency sets')
     S('#pragma unroll 1')
     S(f'for (int s = 0; s < {nsets}; ++s) {{')
@@ -430,6 +401,56 @@ def emit_bk1(t, info, variant='fp64'):
     S('  }')
     S(f'  for (int i = 0; i < mt[4]; ++i) wd[{pre}_nsp[{MS} * r + i]] += {pre}_nnu[{MS} * r + i] * q;')
     S('}')
+
+
+def emit_bk1(t, info, variant='fp64'):
+    """Rolled BK1. variant 'fp64': FP64 exponentials and falloff; 'mixed_valu': Arrhenius
+    exponent in FP32, its sum with the FP64 Gibbs terms through kx_pow2 (FP64 rint, FP32
+    exp2 of the fraction), falloff in FP32; Gibbs terms, net rates and the wdot sums in
+    FP64 -- the straight-line forms' split."""
+    N, R = int(t['n_species']), int(t['n_reactions'])
+    name = f'kx_bk1_{variant}'
+    pre = f'{name}_r'
+    mixed = variant != 'fp64'
+    tabs, nsets, MX, MS = _bk1_tables(t, pre)
+    S = E._Src()
+    for line in tabs:
+        S(line, 0)
+    S(f'__device__ __forceinline__ double {pre}_pow2(double y)', 0)
+    S('{', 0)
+    if mixed:
+        S('const double n = __builtin_rint(y);')
+        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
+        S('return __builtin_ldexp((double)m, (int)n);')
+    else:
+        S('return exp2(y);')
+    S('}', 0)
+    for line in plog_helpers(pre):
+        S(line, 0)
+    S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
+    S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
+    S('            const double pressure_R, const double pressure_,', 0)
+    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
+    S('{', 0)
+    S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
+    S('if (id >= n_states) return;')
+    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
+    S('(void)P; (void)lnP;')
+    S('const double T = Tref * state[id];')
+    S('const double rcpT = 1.0 / T;')
+    S('const double lnT = log(T);')
+    S('const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;')
+    S('const double log2T = lnT * 1.4426950408889634;')
+    S('const float Tf = (float)T, rcpTf = (float)rcpT, log2Tf = (float)log2T;')
+    S('(void)Tf; (void)rcpTf; (void)log2Tf;')
+    S(f'double w[{N}], wd[{N}];   // per-lane private arrays')
+    S('double rcpMbar = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) {{ const double c = fmax(0.0, state[id + offsetT + k * offset]) * {pre}_rcpW[k]; w[k] = c; rcpMbar += c; }}')
+    S('const double rho = pressure_R * rcpT / rcpMbar;')
+    S('#pragma unroll 1')
+    S(f'for (int k = 0; k < {N}; ++k) w[k] *= rho;')
+    emit_bk1_core(S, t, pre, nsets, MX, MS, mixed)
     S('// ---- mass production rates and heat release rate')
     S('double sum_h = 0.0;')
     S('#pragma unroll 1')

@@ -48,3 +48,29 @@ def cond_err(x, y, scale=None):
     with np.errstate(divide='ignore', invalid='ignore'):
         r = np.where(d == 0, 0.0, d / np.where(den == 0, np.finfo(float).tiny, den))
     return float(r.max()) if r.size else 0.0
+
+
+def flux_scale(orc, t, s, p):
+    """Condition scale of BK1 outputs (N + 1, n): the reaction flux each output is a sum of,
+    forward and reverse separately (net rates cancel near equilibrium and at high pressure):
+    W_k sum_r |nu_kr| (q_fwd,r + q_rev,r) for the species rows, R T sum_k flux_k |h_k / RT|
+    for the heat release rate."""
+    q, qf = orc.reaction_rates(s, p, forward=True)
+    net = np.asarray(t['nu_r']) - np.asarray(t['nu_f'])
+    flux = np.abs(net).T @ (np.abs(qf) + np.abs(qf - q))                     # (N, n)
+    T = s[0]
+    powers = np.stack([np.ones_like(T), T, T * T, T ** 3, T ** 4, 1.0 / T])  # (6, n)
+    hi = (T[None, :] > np.asarray(t['T_mid'])[:, None])
+    hRT = np.where(hi, np.asarray(t['h_hi']) @ powers, np.asarray(t['h_lo']) @ powers)
+    hrr = abs(float(t['R'])) * T * (flux * np.abs(hRT)).sum(axis=0)
+    return np.vstack([hrr[None, :], np.asarray(t['W'])[:, None] * flux])
+
+
+def fit_kappa(c, T):
+    """Condition number sum_i |c_i| |ln T|^i / |sum_i c_i ln T^i| of degree-4 fits c (..., 5)
+    at temperatures T (n,): (..., n)."""
+    lT = np.log(T)
+    P = np.stack([lT ** i for i in range(5)])
+    c = np.asarray(c).reshape(-1, 5)
+    c = c[np.any(c != 0.0, axis=1)]                    # pairs without a fit are all zero
+    return (np.abs(c) @ np.abs(P)) / np.abs(c @ P)

@@ -13,7 +13,7 @@ import pytest
 import torch
 
 import kat
-from conftest import ROOT, cond_err
+from conftest import ROOT, cond_err, fit_kappa, flux_scale
 from kinetix_amd import states
 from kinetix_amd.api import Kinetix
 from kinetix_amd.gen import tables
@@ -47,22 +47,6 @@ def _rates(kx, s, p_norm, mixed=False):
     return dr.cpu().numpy().reshape(s.shape)
 
 
-def _flux_scale(orc, t, s, p):
-    """Condition scale of BK1 outputs (N + 1, n): the reaction flux each output is a sum of,
-    forward and reverse separately (net rates cancel near equilibrium and at high pressure):
-    W_k sum_r |nu_kr| (q_fwd,r + q_rev,r) for the species rows, R T sum_k flux_k |h_k / RT|
-    for the heat release rate."""
-    q, qf = orc.reaction_rates(s, p, forward=True)
-    net = np.asarray(t['nu_r']) - np.asarray(t['nu_f'])
-    flux = np.abs(net).T @ (np.abs(qf) + np.abs(qf - q))                     # (N, n)
-    T = s[0]
-    powers = np.stack([np.ones_like(T), T, T * T, T ** 3, T ** 4, 1.0 / T])  # (6, n)
-    hi = (T[None, :] > np.asarray(t['T_mid'])[:, None])
-    hRT = np.where(hi, np.asarray(t['h_hi']) @ powers, np.asarray(t['h_lo']) @ powers)
-    hrr = abs(float(t['R'])) * T * (flux * np.abs(hRT)).sum(axis=0)
-    return np.vstack([hrr[None, :], np.asarray(t['W'])[:, None] * flux])
-
-
 def _l2(x, ref):
     out = [np.linalg.norm(x[k] - ref[k]) / np.linalg.norm(ref[k])
            for k in range(ref.shape[0]) if np.linalg.norm(ref[k]) > 0]
@@ -75,7 +59,7 @@ def test_fp64_rates_vs_oracle(setup, p_atm):
     s, _ = states.sample_any(1024, info['species'], t['W'], seed=states.BASE_SEED + 5)
     out = _rates(kx, s, p_atm * P_ATM / kx.refPressure())
     ref = orc.production_rates(s, p_atm * P_ATM, threads=8)
-    assert cond_err(out, ref, _flux_scale(orc, t, s, p_atm * P_ATM)) < 1e-12
+    assert cond_err(out, ref, flux_scale(orc, t, s, p_atm * P_ATM)) < 1e-12
 
 
 @pytest.mark.parametrize('p_atm', (0.05, 1.0, 20.0))
@@ -87,16 +71,6 @@ def test_mixed_rates_vs_oracle(setup, p_atm):
     assert np.all(np.isfinite(out))
     assert _l2(out[1:], ref[1:]) < 1e-5
     assert np.linalg.norm(out[0] - ref[0]) / np.linalg.norm(ref[0]) < 1e-5
-
-
-def _fit_kappa(c, T):
-    """Condition number sum_i |c_i| |ln T|^i / |sum_i c_i ln T^i| of degree-4 fits c (..., 5)
-    at temperatures T (n,): (..., n)."""
-    lT = np.log(T)
-    P = np.stack([lT ** i for i in range(5)])
-    c = np.asarray(c).reshape(-1, 5)
-    c = c[np.any(c != 0.0, axis=1)]                    # pairs without a fit are all zero
-    return (np.abs(c) @ np.abs(P)) / np.abs(c @ P)
 
 
 def _rel_per_state(a, b):
@@ -120,7 +94,7 @@ def test_transport_and_thermo_vs_oracle(setup):
     torch.cuda.synchronize()
     c0, v0, r0, _ = orc.transport(s, 1.0, threads=8)
     T = s[0]
-    kappa = np.maximum.reduce([_fit_kappa(t[key], T).max(axis=0) for key in ('cond', 'visc', 'diff')])
+    kappa = np.maximum.reduce([fit_kappa(t[key], T).max(axis=0) for key in ('cond', 'visc', 'diff')])
     tol = 1e-12 + 2e-15 * kappa
     assert np.all(_rel_per_state(v.cpu().numpy(), v0) <= tol)
     assert np.all(_rel_per_state(c.cpu().numpy(), c0) <= tol)
@@ -150,7 +124,7 @@ def test_known_answer_states(setup):
         s = kat.state_column(d)
         out = _rates(kx, s, d['P'] / kx.refPressure())
         ref = orc.production_rates(s, d['P'])
-        scale = _flux_scale(orc, t, s, d['P'])
+        scale = flux_scale(orc, t, s, d['P'])
         assert cond_err(out, ref, scale) < 1e-12
         flux = scale[1:, 0] / t['W']
         molar = out[1:, 0] / t['W']
