@@ -62,14 +62,15 @@ typedef struct {
   int fallback;             /* out_of_hull + audit_failed, or n_states after a whole-call fallback */
 } kx_sur_stats;
 
-/* Device workspace (bytes) kx_sur_transport needs for n_states states and this audit stride
- * (with_dkm: D_km requested). Proportional to n_states; no allocation happens in the call. */
+/* Device workspace (bytes) kx_sur_transport needs for n_states states and this audit stride:
+ * counters, per-state hull flags and the queue of state ids, about 8 bytes per state
+ * (with_dkm does not change it). No allocation happens in the call. */
 size_t kx_sur_workspace_bytes(kx_mech_t m, int n_states, int audit_stride, int with_dkm);
 
 /* Audited surrogate transport, the drop-in beside kx_mixture_avg_transport_pstate:
  *  1. kx_sur_infer on every state (out-of-hull states queued);
  *  2. the plain FP64 kernel on every audit_stride-th state (from audit_offset % audit_stride;
- *     audit_stride <= 0: no audit) and on the queued states, from a compact copy, and its
+ *     audit_stride <= 0: no audit) and on the queued states, reading them in place, and its
  *     results written over theirs (audit samples are compared first: relative error of mu
  *     or lambda above audit_tol, or NaN, counts as a failure);
  *  3. if failures exceed max_audit_fail * audited, the plain kernel for all states.

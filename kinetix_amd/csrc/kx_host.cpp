@@ -133,7 +133,7 @@ static int finish_init(Mech* m, kx_mech_t* out)
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
-                        "kx_sur_queue_init", "kx_sur_gather", "kx_sur_merge",
+                        "kx_sur_queue_init", "kx_bk2_fp64_queue",
                         "kx_bk1_routines", "kx_bk2_routines"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) != hipSuccess) {
@@ -417,33 +417,26 @@ int kx_sur_infer(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
 namespace {
 
 struct SurPlan {
-  long long n_audit, audit_max, cap, slots;
-  size_t counters, flags, queue, cstate, cp, cvisc, ccond, crhoD, cDkm, total;
+  long long n_audit, audit_max, cap;
+  size_t counters, flags, queue, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-SurPlan sur_plan(int n, int N, int audit_stride, int audit_offset, int with_dkm)
+SurPlan sur_plan(int n, int audit_stride, int audit_offset)
 {
   SurPlan P{};
   const long long stride = audit_stride > 0 ? audit_stride : 0;
   const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
   P.audit_max = stride ? (n + stride - 1) / stride : 0;
   P.n_audit = stride && n > a0 ? (n - a0 + stride - 1) / stride : 0;
-  // out-of-hull states handled per compact pass: 1/64 of the call (at least 4096)
+  // out-of-hull states handled per plain pass: 1/64 of the call (at least 4096)
   P.cap = std::min<long long>(n, std::max<long long>(4096, n / 64));
-  P.slots = P.audit_max + P.cap;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
   P.counters = take(4 * sizeof(int));
   P.flags = take((size_t)n * sizeof(int));
   P.queue = take((size_t)(P.audit_max + n) * sizeof(int));
-  P.cstate = take((size_t)(N + 1) * P.slots * sizeof(double));
-  P.cp = take((size_t)P.slots * sizeof(double));
-  P.cvisc = take((size_t)P.slots * sizeof(double));
-  P.ccond = take((size_t)P.slots * sizeof(double));
-  P.crhoD = take((size_t)N * P.slots * sizeof(double));
-  P.cDkm = with_dkm ? take((size_t)N * P.slots * sizeof(double)) : 0;
   P.total = o;
   return P;
 }
@@ -454,7 +447,8 @@ size_t kx_sur_workspace_bytes(kx_mech_t h, int n, int audit_stride, int with_dkm
 {
   Mech* m = get(h);
   if (!m || n < 0) return 0;
-  return sur_plan(n, m->n_species, audit_stride, 0, with_dkm).total;
+  (void)with_dkm;   // the plain pass writes D_km in place: no staging either way
+  return sur_plan(n, audit_stride, 0).total;
 }
 
 int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
@@ -470,11 +464,11 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
   if (n > 0 && (!d_norm || !d_ostat || !d_wfrag || !d_params || !d_visc || !d_cond || !d_rhoD))
     return fail(KX_ERR_INVALID_ARG, "null device pointer");
   if (n > 0 && !d_work) return fail(KX_ERR_INVALID_ARG, "null workspace");
-  const int N = m->n_species;
-  const SurPlan P = sur_plan(n, N, audit_stride, audit_offset, d_Dkm != nullptr);
+  const SurPlan P = sur_plan(n, audit_stride, audit_offset);
+  if (P.audit_max + (long long)n > INT32_MAX)
+    return fail(KX_ERR_TOO_LARGE, "too many states for one call");
   if (work_bytes < P.total)
     return fail(KX_ERR_TOO_LARGE, "workspace smaller than kx_sur_workspace_bytes()");
-  if (P.slots > INT32_MAX / (N + 1)) return fail(KX_ERR_TOO_LARGE, "too many states for one call");
   kx_sur_stats st{};
   st.n_states = n;
   if (n == 0) { if (stats) *stats = st; return KX_OK; }
@@ -482,15 +476,9 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
   int* counters = reinterpret_cast<int*>(w + P.counters);
   int* flags = reinterpret_cast<int*>(w + P.flags);
   int* queue = reinterpret_cast<int*>(w + P.queue);
-  double* cstate = reinterpret_cast<double*>(w + P.cstate);
-  double* cp = reinterpret_cast<double*>(w + P.cp);
-  double* cvisc = reinterpret_cast<double*>(w + P.cvisc);
-  double* ccond = reinterpret_cast<double*>(w + P.ccond);
-  double* crhoD = reinterpret_cast<double*>(w + P.crhoD);
-  double* cDkm = d_Dkm ? reinterpret_cast<double*>(w + P.cDkm) : nullptr;
   const long long stride = audit_stride > 0 ? audit_stride : 0;
   const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
-  int n_audit = (int)P.n_audit, slots = (int)P.slots, cap = (int)P.cap;
+  int n_audit = (int)P.n_audit, cap = (int)P.cap;
 
   // 1. audit ids and counters
   {
@@ -502,21 +490,15 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
   if ((rc = kx_sur_infer(h, n, offsetT, offset, d_p, d_state, d_norm, d_ostat, d_wfrag, d_params,
                          flags, d_visc, d_cond, d_rhoD, d_Dkm, fbq, counters, stream)))
     return rc;
-  // 3. plain kernel on the audit samples + queued out-of-hull states, then merge
+  // 3. plain kernel over the audit samples + queued out-of-hull states, in place: it reads
+  //    the queued states, compares mu and lambda of the audit entries with the surrogate's,
+  //    and overwrites the outputs with its own (the queue length is read on the device)
   auto compact_pass = [&](int n_aud, int* q, int skip) -> int {
     long long oT = offsetT, o = offset;
-    {
-      void* args[] = {&slots, &n_aud, &cap, &skip, &q, &counters, &oT, &o, (void*)&d_p,
-                      (void*)&d_state, &cstate, &cp};
-      if (int r = launch(m, "kx_sur_gather", slots, stream, args)) return r;
-    }
-    if (int r = transport_impl(h, slots, slots, slots, 1.0, cp, cstate, cvisc, ccond, crhoD, cDkm,
-                               stream, counters + 2))
-      return r;
-    long long oo = offset;
-    void* args[] = {&slots, &n_aud, &q, &counters, &flags, &oo, &audit_tol, &cvisc, &ccond,
-                    &crhoD, &cDkm, &d_visc, &d_cond, &d_rhoD, &d_Dkm};
-    return launch(m, "kx_sur_merge", slots, stream, args);
+    double Tref = m->ref_T;
+    void* args[] = {&n_aud, &cap, &skip, &q, &counters, &flags, &audit_tol, &oT, &o,
+                    (void*)&d_p, (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
+    return launch(m, "kx_bk2_fp64_queue", n_aud + cap, stream, args);
   };
   if ((rc = compact_pass(n_audit, queue, 0))) return rc;
   int host_counters[4] = {0, 0, 0, 0};

@@ -68,14 +68,29 @@ class ConstPool:
             return ''
         n = len(self.values)
         return (f'  __shared__ double {self.ref}[{n}];\n'
-                f'  for (int i_ = threadIdx.x; i_ < {n}; i_ += {block}) {self.ref}[i_] = {self.name}[i_];\n'
-                f'  __syncthreads();\n')
+                + ''.join('  ' + line + '\n' for line in lds_stage(self.ref, self.name, n, block))
+                + '  __syncthreads();\n')
 
     def declaration(self):
         if not self.values:
             return ''
         body = ', '.join(_literal(v) for v in self.values)
         return f'__device__ double {self.name}[{len(self.values)}] = {{{body}}};\n'
+
+
+def lds_stage(dst, src, n, block, batch=16):
+    """Lines of C++ copying src[0, n) (global) to dst (LDS) with `block` threads: each thread
+    issues `batch` loads (indices clamped in bounds, so they form one block) before its LDS
+    writes. A plain strided copy waits out one L2 round trip per element and thread: ~20 us
+    for a 10^4-entry table, idle time for every workgroup the table's LDS keeps alone on a CU."""
+    step = batch * block
+    return [f'for (int b_ = threadIdx.x; b_ < {n}; b_ += {step}) {{',
+            f'  double t_[{batch}];',
+            '  #pragma unroll',
+            f'  for (int r_ = 0; r_ < {batch}; ++r_) t_[r_] = {src}[min(b_ + r_ * {block}, {n - 1})];',
+            '  #pragma unroll',
+            f'  for (int r_ = 0; r_ < {batch}; ++r_) if (b_ + r_ * {block} < {n}) {dst}[b_ + r_ * {block}] = t_[r_];',
+            '}']
 
 
 _POOL = None
@@ -704,7 +719,8 @@ def _emit_bk1_live(t, info, variant='fp64', arrh32=False, order=None, waves=0):
     S('{', 0)
     S(POOL_MARK)
     S(f'__shared__ double poly_s[{PT.size}];')
-    S(f'for (int i_ = threadIdx.x; i_ < {PT.size}; i_ += {BLOCK}) poly_s[i_] = {kname}_poly[i_];')
+    for line in lds_stage('poly_s', f'{kname}_poly', PT.size, BLOCK):
+        S(line)
     S('__syncthreads();')
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
@@ -803,37 +819,62 @@ def _poly4(c):
             f'{lit(c[4])} * lnT4')
 
 
-def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK):
+def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     """pstate=False: kx_bk2_fp64, scalar p (the reference's transport kernel).
     pstate=True: kx_bk2_fp64_pstate, p [Pa] per state, also writes D_km.
+    queue=True: kx_bk2_fp64_queue, the plain kernel over a device queue of state ids (the
+    surrogate's audit samples, then its out-of-hull states), p per state, reading each state
+    and writing its outputs in place; audit entries first compare mu and lambda with the
+    surrogate values they replace.
     block: threads per workgroup (the host launches with the kernel's launch bound). The
     kernel's LDS constant pool (~85 KB for GRI-3.0) admits one workgroup per CU, so 512
     threads (8 waves) is what gives 2 waves/SIMD: 8.4 ms instead of 14.5 at 256 threads for
     4M states, despite ~160 VGPRs spilled to scratch; 768 and 1024 spill far more and are
-    2-3x slower."""
+    2-3x slower. The queue form is latency-bound (a few 10^4 states): 256 threads, one wave per
+    SIMD on every CU, each wave free to use 372 of the 512 registers without a spill."""
     N = int(t['n_species'])
     rcp_diff = bool(int(t.get('rcp_diff', 0)))
     S = _Src()
     S(f'extern "C" __global__ void __launch_bounds__({block})', 0)
-    if pstate:
+    if queue:
+        S('kx_bk2_fp64_queue(const int n_audit, const int cap, const int skip,', 0)
+        S('            const int* __restrict__ queue, int* __restrict__ counters,', 0)
+        S('            const int* __restrict__ flags, const double tol,', 0)
+        S('            const long long offsetT, const long long offset,', 0)
+        S('            const double* __restrict__ p_state,', 0)
+        S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
+        S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
+        S('            double* __restrict__ Dkm, const double Tref)', 0)
+        S('{', 0)
+        S('// queue entries [0, n_audit) are audit samples, then min(counters[0] - skip, cap)')
+        S('// out-of-hull states; counters: [0] out-of-hull states queued, [1] failed audits')
+        S('const int valid = n_audit + min(max(counters[0] - skip, 0), cap);')
+        S(f'if ((long long)blockIdx.x * {block} >= valid) return;   // whole workgroup past the queue')
+        S(POOL_MARK)
+        S(f'const long long slot = (long long)blockIdx.x * {block} + threadIdx.x;')
+        S('if (slot >= valid) return;')
+        S('const long long id = queue[slot];')
+        S('const double p = p_state[id];')
+    elif pstate:
         S('kx_bk2_fp64_pstate(const int n_states, const long long offsetT, const long long offset,', 0)
         S('            const double* __restrict__ p_state,', 0)
     else:
         S('kx_bk2_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
         S('            const double pressure,', 0)
-    S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
-    S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
-    if pstate:
-        S('            double* __restrict__ Dkm, const double Tref,', 0)
-    else:
-        S('            const double Tref,', 0)
-    S('            const int* __restrict__ n_dev)', 0)
-    S('{', 0)
-    S(POOL_MARK)
-    S(f'const long long id = (long long)blockIdx.x * {block} + threadIdx.x;')
-    S('// n_dev (optional): a state count known only on the device (audit/fallback queues)')
-    S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
-    S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
+    if not queue:
+        S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
+        S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
+        if pstate:
+            S('            double* __restrict__ Dkm, const double Tref,', 0)
+        else:
+            S('            const double Tref,', 0)
+        S('            const int* __restrict__ n_dev)', 0)
+        S('{', 0)
+        S(POOL_MARK)
+        S(f'const long long id = (long long)blockIdx.x * {block} + threadIdx.x;')
+        S('// n_dev (optional): a state count known only on the device')
+        S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
+        S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
     S('const double T = Tref * state[id];')
     S('const double lnT = log(T);')
     S('const double rcpT = 1.0 / T;')
@@ -863,9 +904,20 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK):
     S('const double TsqrT = T * sqrT;')
     S(f'const double rho = p * {lit(1.0 / float(t["R"]))} * rcpT * Mbar;')
     S('const double pMbar = p * Mbar;')
-    for k in range(N):
-        S(f'{{ const double D = TsqrT * (Mbar - {lit(W[k])} * x{k}) / (pMbar * s{k});')
-        S(f'  rhoD[{k} * offset + id] = rho * D;' + (f' Dkm[{k} * offset + id] = D; }}' if pstate else ' }'))
+    def stores(with_dkm):
+        for k in range(N):
+            S(f'{{ const double D = TsqrT * (Mbar - {lit(W[k])} * x{k}) / (pMbar * s{k});')
+            S(f'  rhoD[{k} * offset + id] = rho * D;' + (f' Dkm[{k} * offset + id] = D; }}' if with_dkm else ' }'))
+    if queue:
+        # D_km optional: one uniform branch around the stores (a branch per store splits the
+        # kernel into ~10^2 blocks, and the scheduler then spills ~10^4 VGPRs)
+        S('if (Dkm) {')
+        stores(True)
+        S('} else {')
+        stores(False)
+        S('}')
+    else:
+        stores(pstate)
     # viscosity: Wilke, eta_k = poly, Phi^(1/2)_kj = C1 + C2 eta_k / eta_j
     S('__builtin_amdgcn_sched_barrier(0);')
     S('// ---- viscosity (Wilke)')
@@ -883,12 +935,22 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK):
         S(f'  vis += x{k} * e * e / s;')
         S('}')
     S('__builtin_amdgcn_sched_barrier(0);')
-    S('viscosity[id] = sqrT * vis;')
+    S('const double mu = sqrT * vis;')
+    if not queue:
+        S('viscosity[id] = mu;')
     S('// ---- conductivity')
     S('double sum1 = 0.0, sum2 = 0.0;')
     for k in range(N):
         S(f'{{ const double l = {_poly4(t["cond"][k])}; sum1 += x{k} * l; sum2 += x{k} / l; }}')
-    S('conductivity[id] = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+    S('const double lambda = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+    if queue:
+        S('if (slot < n_audit && flags[id] == 0) {   // audit of a state the surrogate answered')
+        S('  const double ev = fabs(viscosity[id] / mu - 1.0);')
+        S('  const double ec = fabs(conductivity[id] / lambda - 1.0);')
+        S('  if (!(ev <= tol) || !(ec <= tol)) atomicAdd(counters + 1, 1);')
+        S('}')
+        S('viscosity[id] = mu;')
+    S('conductivity[id] = lambda;')
     S('}', 0)
     return S.text()
 
@@ -998,6 +1060,8 @@ def generate(tables, info, opts=None):
                                 lds=o['lds_pool'], block=o['bk2_block']))
         parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, o['bk2_block'],
                                 lds=o['lds_pool'], block=o['bk2_block']))
+        parts.append(_with_pool('kx_bk2_fp64_queue_k', _emit_bk2_fp64, tables, info, True,
+                                256, True, lds=o['lds_pool'], block=256))
     parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
     from . import emit_mixed
     parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],

@@ -323,9 +323,10 @@ def emit(t, info):
     S('}')
     S('}', 0)
 
-    # ---------------- audit / fallback plumbing (queue of state ids -> compact plain run)
-    S('// counters: [0] states queued as out of hull, [1] failed audits, [2] valid queue entries')
-    S('// of the current compact pass')
+    # ---------------- audit / fallback plumbing: a queue of state ids for the plain kernel
+    S('// counters: [0] states queued as out of hull, [1] failed audits. The queue holds the')
+    S('// audit samples a0 + i * stride, then the out-of-hull ids kx_sur_infer appends; the plain')
+    S('// kernel kx_bk2_fp64_queue runs over it in place.')
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
     S('kx_sur_queue_init(const int n_audit, const long long a0, const long long stride,', 0)
     S('                  int* __restrict__ queue, int* __restrict__ counters)', 0)
@@ -333,47 +334,5 @@ def emit(t, info):
     S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (i < 4) counters[i] = 0;')
     S('if (i < n_audit) queue[i] = (int)(a0 + i * stride);')
-    S('}', 0)
-    S('// Compact SoA copy (offsetT = offset = slots) of the queued states: entries [0, n_audit)')
-    S('// are audit samples, then min(counters[0] - skip, cap) out-of-hull states.')
-    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
-    S('kx_sur_gather(const int slots, const int n_audit, const int cap, const int skip,', 0)
-    S('              const int* __restrict__ queue, int* __restrict__ counters,', 0)
-    S('              const long long offsetT, const long long offset,', 0)
-    S('              const double* __restrict__ p_state, const double* __restrict__ state,', 0)
-    S('              double* __restrict__ cstate, double* __restrict__ cp)', 0)
-    S('{', 0)
-    S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
-    S('if (i >= slots) return;')
-    S('const int valid = n_audit + min(max(counters[0] - skip, 0), cap);')
-    S('if (i == 0) counters[2] = valid;')
-    S('if (i >= valid) return;')
-    S('const long long sid = queue[i];')
-    S('cstate[i] = state[sid];')
-    S(f'for (int k = 0; k < {N}; ++k) cstate[i + slots + k * (long long)slots] = state[sid + offsetT + k * offset];')
-    S('cp[i] = p_state[sid];')
-    S('}', 0)
-    S('// Plain results of the compact pass back into the outputs; audit entries (state not')
-    S('// flagged) first compare mu and lambda with the surrogate values they replace.')
-    S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
-    S('kx_sur_merge(const int slots, const int n_audit, const int* __restrict__ queue,', 0)
-    S('             int* __restrict__ counters, const int* __restrict__ flags, const long long offset,', 0)
-    S('             const double tol, const double* __restrict__ cvisc, const double* __restrict__ ccond,', 0)
-    S('             const double* __restrict__ crhoD, const double* __restrict__ cDkm,', 0)
-    S('             double* __restrict__ viscosity, double* __restrict__ conductivity,', 0)
-    S('             double* __restrict__ rhoD, double* __restrict__ Dkm)', 0)
-    S('{', 0)
-    S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
-    S('if (i >= slots || i >= counters[2]) return;')
-    S('const long long sid = queue[i];')
-    S('if (i < n_audit && flags[sid] == 0) {')
-    S('  const double ev = fabs(viscosity[sid] / cvisc[i] - 1.0);')
-    S('  const double ec = fabs(conductivity[sid] / ccond[i] - 1.0);')
-    S('  if (!(ev <= tol) || !(ec <= tol)) atomicAdd(counters + 1, 1);')
-    S('}')
-    S('viscosity[sid] = cvisc[i];')
-    S('conductivity[sid] = ccond[i];')
-    S(f'for (int k = 0; k < {N}; ++k) rhoD[k * offset + sid] = crhoD[k * (long long)slots + i];')
-    S(f'if (Dkm) for (int k = 0; k < {N}; ++k) Dkm[k * offset + sid] = cDkm[k * (long long)slots + i];')
     S('}', 0)
     return S.text()

@@ -31,8 +31,7 @@ def test_emitted_kernels_and_counts():
     src = open(os.path.join(MDIR, 'kernels.hip')).read()
     for k in ('kx_bk1_fp64', 'kx_bk1_mixed', 'kx_bk1_mixed_valu', 'kx_bk2_fp64',
               'kx_bk2_fp64_pstate', 'kx_thermo_fp64', 'kx_sur_features', 'kx_sur_targets',
-              'kx_sur_outputs', 'kx_sur_infer', 'kx_sur_queue_init', 'kx_sur_gather',
-              'kx_sur_merge'):
+              'kx_sur_outputs', 'kx_sur_infer', 'kx_sur_queue_init', 'kx_bk2_fp64_queue'):
         assert f'\n{k}(' in src, k
     counts = json.load(open(os.path.join(MDIR, 'counts.json')))
     assert counts['kx_bk1_fp64']['bytes'] == 16 * 54
