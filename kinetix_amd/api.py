@@ -100,7 +100,13 @@ def _check_tensor(name, t, numel=None):
 
 
 def mech_dir(name):
-    """Directory of a generated mechanism shipped with the package (e.g. 'gri30')."""
+    """Directory of a generated mechanism: one shipped with the package (e.g. 'gri30'), a
+    directory, or a Cantera YAML file, generated and compiled into the build cache
+    (kinetix_amd.cache: .cache/KINETIX/<stem>, regenerated only when its .hash is stale, as
+    the reference's init does)."""
+    if name.endswith(('.yaml', '.yml')) and os.path.isfile(name):
+        from . import cache
+        return cache.ensure(name)
     d = name if os.path.isdir(name) else os.path.join(MECH_ROOT, name)
     if not os.path.exists(os.path.join(d, 'kernels.hsaco')):
         raise KinetixError(f'no built code object in {d} (python -m kinetix_amd.build)')
