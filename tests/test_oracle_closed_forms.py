@@ -148,3 +148,48 @@ def test_pure_species_and_binary_transport_limits(oracle, gri):
     d = np.asarray(t['diff'])[a, b] @ P
     D12 = (T ** 1.5 / (d * p)) if int(t.get('rcp_diff', 0)) else d * T ** 1.5 / p
     np.testing.assert_allclose(Dkm[i1], D12 * W[i2] / Mbar, rtol=1e-12)
+
+
+# PLOG (Cantera's published semantics, the reference generator's reaction_rates.py:262-390):
+# at a listed pressure P_i, k(P_i) = sum of the Arrhenius terms given for P_i; between listed
+# pressures ln k is linear in ln P; outside the range, the end point's k.
+PLOG_POINTS = [(0.1, [{'A': 2.0e12, 'b': 0.3, 'Ea': 60000.0}]),
+               (1.0, [{'A': 5.0e12, 'b': 0.1, 'Ea': 58000.0}, {'A': 4.0e9, 'b': 1.2, 'Ea': 45000.0}]),
+               (10.0, [{'A': 9.0e13, 'b': -0.4, 'Ea': 61000.0}])]
+
+
+def _plog_model():
+    sp = [{'name': n, 'composition': c, 'thermo': NASA}
+          for n, c in (('H2', {'H': 2}), ('H', {'H': 1}), ('N2', {'N': 2}))]
+    rcs = [dict(P=f'{p} atm', **r) for p, rates in PLOG_POINTS for r in rates]
+    rx = [{'equation': 'H2 => 2 H', 'type': 'pressure-dependent-Arrhenius', 'rate-constants': rcs}]
+    return {'units': {'length': 'cm', 'time': 's', 'quantity': 'mol', 'activation-energy': 'cal/mol'},
+            'species': sp, 'reactions': rx}
+
+
+@pytest.mark.parametrize('T', [900.0, 1700.0, 2400.0])
+@pytest.mark.parametrize('p_atm', [0.01, 0.1, 0.3, 1.0, 3.0, 10.0, 100.0])
+def test_plog_summed_points_and_interpolation(T, p_atm):
+    """A PLOG reaction whose 1 atm point sums two Arrhenius terms: the oracle's k against the
+    closed form below, on, between and beyond the listed pressures."""
+    t = tables.build(mechanism.from_model(_plog_model(), 'plogmini'))
+    orc = po.Oracle(t)
+    Y = np.array([0.2, 0.01, 0.79])
+    s = np.zeros((4, 1)); s[0, 0] = T; s[1:, 0] = Y
+    p = p_atm * 101325.0
+    q = orc.reaction_rates(s, p)[0, 0]
+    k = [sum(_arr(r, T, 0) for r in rates) for _, rates in PLOG_POINTS]
+    lnP = [math.log(pp) for pp, _ in PLOG_POINTS]
+    x = math.log(p_atm)
+    if x <= lnP[0]:
+        lnk = math.log(k[0])
+    elif x >= lnP[-1]:
+        lnk = math.log(k[-1])
+    else:
+        i = max(j for j in range(len(lnP) - 1) if lnP[j] <= x)
+        lnk = math.log(k[i]) + (math.log(k[i + 1]) - math.log(k[i])) * (x - lnP[i]) / (lnP[i + 1] - lnP[i])
+    expect = math.exp(lnk) * _conc(t, T, Y, p)[0]
+    assert q == pytest.approx(expect, rel=1e-12, abs=0)
+    # the summed point is not either term alone
+    if p_atm == 1.0:
+        assert abs(q / (_arr(PLOG_POINTS[1][1][0], T, 0) * _conc(t, T, Y, p)[0]) - 1) > 1e-3
