@@ -133,7 +133,7 @@ static int finish_init(Mech* m, kx_mech_t* out)
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
                         "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
-                        "kx_sur_queue_init", "kx_bk2_fp64_queue",
+                        "kx_sur_queue_init", "kx_bk2_fp64_queue", "kx_sur_decide",
                         "kx_bk1_routines", "kx_bk2_routines"}) {
     hipFunction_t f = nullptr;
     if (hipModuleGetFunction(&f, m->module, k) != hipSuccess) {
@@ -430,7 +430,8 @@ SurPlan sur_plan(int n, int audit_stride, int audit_offset)
   const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
   P.audit_max = stride ? (n + stride - 1) / stride : 0;
   P.n_audit = stride && n > a0 ? (n - a0 + stride - 1) / stride : 0;
-  // out-of-hull states handled per plain pass: 1/64 of the call (at least 4096)
+  // grid of the plain pass: the audit samples + 1/64 of the call (at least 4096) threads; it
+  // strides over a longer queue
   P.cap = std::min<long long>(n, std::max<long long>(4096, n / 64));
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
@@ -492,36 +493,38 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
     return rc;
   // 3. plain kernel over the audit samples + queued out-of-hull states, in place: it reads
   //    the queued states, compares mu and lambda of the audit entries with the surrogate's,
-  //    and overwrites the outputs with its own (the queue length is read on the device)
-  auto compact_pass = [&](int n_aud, int* q, int skip) -> int {
+  //    and overwrites the outputs with its own. The queue length is read on the device and
+  //    the grid strides over it, so one launch covers any number of out-of-hull states.
+  {
     long long oT = offsetT, o = offset;
     double Tref = m->ref_T;
-    void* args[] = {&n_aud, &cap, &skip, &q, &counters, &flags, &audit_tol, &oT, &o,
-                    (void*)&d_p, (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
-    return launch(m, "kx_bk2_fp64_queue", n_aud + cap, stream, args);
-  };
-  if ((rc = compact_pass(n_audit, queue, 0))) return rc;
-  int host_counters[4] = {0, 0, 0, 0};
-  KX_HIP(hipMemcpyAsync(host_counters, counters, sizeof(host_counters), hipMemcpyDeviceToHost,
-                        (hipStream_t)stream));
-  KX_HIP(hipStreamSynchronize((hipStream_t)stream));
-  st.out_of_hull = host_counters[0];
-  st.audited = n_audit;
-  st.audit_failed = host_counters[1];
-  if (n_audit > 0 && host_counters[1] > max_audit_fail * n_audit) {
-    // the model disagrees with the plain kernel on too many samples: plain kernel for all
-    st.whole_call_fallback = 1;
-    st.fallback = n;
-    if ((rc = transport_impl(h, n, offsetT, offset, 1.0, d_p, d_state, d_visc, d_cond, d_rhoD,
-                             d_Dkm, stream)))
-      return rc;
-  } else {
-    // out-of-hull states beyond the first pass's capacity, cap at a time
-    for (int done = cap; done < st.out_of_hull; done += cap)
-      if ((rc = compact_pass(0, queue + n_audit + done, done))) return rc;
-    st.fallback = st.out_of_hull + st.audit_failed;
+    void* args[] = {&n_audit, &queue, &counters, &flags, &audit_tol, &oT, &o, (void*)&d_p,
+                    (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
+    if ((rc = launch(m, "kx_bk2_fp64_queue", n_audit + cap, stream, args))) return rc;
   }
-  if (stats) *stats = st;
+  // 4. whole-call fallback, decided on the device: counters[3] = n when the failed audits
+  //    exceed max_audit_fail * n_audit, else 0; the plain kernel for all states runs with
+  //    that device-side count (every workgroup returns at once when it is 0)
+  {
+    void* args[] = {&n, &n_audit, &max_audit_fail, &counters};
+    if ((rc = launch(m, "kx_sur_decide", 1, stream, args))) return rc;
+  }
+  if ((rc = transport_impl(h, n, offsetT, offset, 1.0, d_p, d_state, d_visc, d_cond, d_rhoD,
+                           d_Dkm, stream, counters + 3)))
+    return rc;
+  // 5. statistics: the only host read, and the only wait, of the call
+  if (stats) {
+    int host_counters[4] = {0, 0, 0, 0};
+    KX_HIP(hipMemcpyAsync(host_counters, counters, sizeof(host_counters), hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    KX_HIP(hipStreamSynchronize((hipStream_t)stream));
+    st.out_of_hull = host_counters[0];
+    st.audited = n_audit;
+    st.audit_failed = host_counters[1];
+    st.whole_call_fallback = host_counters[3] != 0;
+    st.fallback = st.whole_call_fallback ? n : st.out_of_hull + st.audit_failed;
+    *stats = st;
+  }
   return KX_OK;
 }
 

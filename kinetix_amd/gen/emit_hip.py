@@ -837,7 +837,7 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     S = _Src()
     S(f'extern "C" __global__ void __launch_bounds__({block})', 0)
     if queue:
-        S('kx_bk2_fp64_queue(const int n_audit, const int cap, const int skip,', 0)
+        S('kx_bk2_fp64_queue(const int n_audit,', 0)
         S('            const int* __restrict__ queue, int* __restrict__ counters,', 0)
         S('            const int* __restrict__ flags, const double tol,', 0)
         S('            const long long offsetT, const long long offset,', 0)
@@ -846,13 +846,14 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
         S('            double* __restrict__ viscosity, double* __restrict__ rhoD,', 0)
         S('            double* __restrict__ Dkm, const double Tref)', 0)
         S('{', 0)
-        S('// queue entries [0, n_audit) are audit samples, then min(counters[0] - skip, cap)')
-        S('// out-of-hull states; counters: [0] out-of-hull states queued, [1] failed audits')
-        S('const int valid = n_audit + min(max(counters[0] - skip, 0), cap);')
+        S('// queue entries [0, n_audit) are audit samples, then counters[0] out-of-hull states;')
+        S('// counters: [0] out-of-hull states queued, [1] failed audits. The grid strides over')
+        S('// the queue, so one launch of any size covers it.')
+        S('const long long valid = (long long)n_audit + max(counters[0], 0);')
         S(f'if ((long long)blockIdx.x * {block} >= valid) return;   // whole workgroup past the queue')
         S(POOL_MARK)
-        S(f'const long long slot = (long long)blockIdx.x * {block} + threadIdx.x;')
-        S('if (slot >= valid) return;')
+        S(f'for (long long slot = (long long)blockIdx.x * {block} + threadIdx.x; slot < valid;')
+        S(f'     slot += (long long)gridDim.x * {block}) {{')
         S('const long long id = queue[slot];')
         S('const double p = p_state[id];')
     elif pstate:
@@ -870,9 +871,11 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
             S('            const double Tref,', 0)
         S('            const int* __restrict__ n_dev)', 0)
         S('{', 0)
+        S('// n_dev (optional): a state count known only on the device; workgroups wholly past it')
+        S('// return before staging the LDS table')
+        S(f'if (n_dev && (long long)blockIdx.x * {block} >= *n_dev) return;')
         S(POOL_MARK)
         S(f'const long long id = (long long)blockIdx.x * {block} + threadIdx.x;')
-        S('// n_dev (optional): a state count known only on the device')
         S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
         S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
     S('const double T = Tref * state[id];')
@@ -951,6 +954,8 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
         S('}')
         S('viscosity[id] = mu;')
     S('conductivity[id] = lambda;')
+    if queue:
+        S('}   // slot')
     S('}', 0)
     return S.text()
 

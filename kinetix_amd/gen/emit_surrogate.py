@@ -324,7 +324,8 @@ def emit(t, info):
     S('}', 0)
 
     # ---------------- audit / fallback plumbing: a queue of state ids for the plain kernel
-    S('// counters: [0] states queued as out of hull, [1] failed audits. The queue holds the')
+    S('// counters: [0] states queued as out of hull, [1] failed audits, [3] the whole-call')
+    S('// fallback count (kx_sur_decide). The queue holds the')
     S('// audit samples a0 + i * stride, then the out-of-hull ids kx_sur_infer appends; the plain')
     S('// kernel kx_bk2_fp64_queue runs over it in place.')
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
@@ -334,5 +335,14 @@ def emit(t, info):
     S(f'const long long i = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (i < 4) counters[i] = 0;')
     S('if (i < n_audit) queue[i] = (int)(a0 + i * stride);')
+    S('}', 0)
+    S('// The whole-call fallback decided on the device: counters[3] = n_states when the failed')
+    S('// audits exceed max_fail * n_audit, else 0; the plain kernel then runs with that count.')
+    S(f'extern "C" __global__ void __launch_bounds__(64)', 0)
+    S('kx_sur_decide(const int n_states, const int n_audit, const double max_fail,', 0)
+    S('              int* __restrict__ counters)', 0)
+    S('{', 0)
+    S('if (threadIdx.x == 0)')
+    S('  counters[3] = (n_audit > 0 && (double)counters[1] > max_fail * (double)n_audit) ? n_states : 0;')
     S('}', 0)
     return S.text()

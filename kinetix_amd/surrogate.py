@@ -237,9 +237,10 @@ class Surrogate:
 
     # ---------------------------------------------------------------- inference
     def infer(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None,
-              audit_stride=100, audit_tol=1e-3, max_audit_fail=0.01, audit_offset=0):
+              audit_stride=100, audit_tol=1e-3, max_audit_fail=0.01, audit_offset=0, stats=True):
         """Audited surrogate transport for n states (FP64 outputs in the plain kernel's layout),
-        one call of kx_sur_transport. Returns the fallback statistics."""
+        one call of kx_sur_transport. Returns the fallback statistics; stats=False enqueues
+        the call on the current stream without waiting and returns None (same device work)."""
         import torch
         need = int(self.L.kx_sur_workspace_bytes(self.kx.h, n, audit_stride, d_Dkm is not None))
         if getattr(self, '_work', None) is None or self._work.numel() < need:
@@ -249,7 +250,9 @@ class Surrogate:
                                        _p(self.norm), _p(self.ostat), _p(self.wbf16), _p(self.params),
                                        audit_stride, audit_offset, audit_tol, max_audit_fail,
                                        _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm), _p(self._work),
-                                       self._work.numel(), C.byref(st), _s()))
+                                       self._work.numel(), C.byref(st) if stats else None, _s()))
+        if not stats:
+            return None
         return {'n': st.n_states, 'out_of_hull': st.out_of_hull, 'audited': st.audited,
                 'audit_failed': st.audit_failed, 'whole_call_fallback': bool(st.whole_call_fallback),
                 'fallback': st.fallback, 'fallback_fraction': st.fallback / n if n else 0.0}
