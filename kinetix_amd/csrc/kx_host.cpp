@@ -417,7 +417,7 @@ int kx_sur_infer(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
 namespace {
 
 struct SurPlan {
-  long long n_audit, audit_max, cap;
+  long long n_audit, audit_max;
   size_t counters, flags, queue, total;
 };
 
@@ -430,9 +430,6 @@ SurPlan sur_plan(int n, int audit_stride, int audit_offset)
   const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
   P.audit_max = stride ? (n + stride - 1) / stride : 0;
   P.n_audit = stride && n > a0 ? (n - a0 + stride - 1) / stride : 0;
-  // grid of the plain pass: the audit samples + 1/64 of the call (at least 4096) threads; it
-  // strides over a longer queue
-  P.cap = std::min<long long>(n, std::max<long long>(4096, n / 64));
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
   P.counters = take(4 * sizeof(int));
@@ -479,7 +476,7 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
   int* queue = reinterpret_cast<int*>(w + P.queue);
   const long long stride = audit_stride > 0 ? audit_stride : 0;
   const long long a0 = stride ? (long long)(audit_offset < 0 ? 0 : audit_offset) % stride : 0;
-  int n_audit = (int)P.n_audit, cap = (int)P.cap;
+  int n_audit = (int)P.n_audit;
 
   // 1. audit ids and counters
   {
@@ -493,14 +490,15 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
     return rc;
   // 3. plain kernel over the audit samples + queued out-of-hull states, in place: it reads
   //    the queued states, compares mu and lambda of the audit entries with the surrogate's,
-  //    and overwrites the outputs with its own. The queue length is read on the device and
-  //    the grid strides over it, so one launch covers any number of out-of-hull states.
+  //    and overwrites the outputs with its own. The queue length is read on the device; the
+  //    grid covers the longest possible queue (n_audit + n), and the workgroups past the
+  //    actual end return at once, after the working ones (the first) were dispatched.
   {
     long long oT = offsetT, o = offset;
     double Tref = m->ref_T;
     void* args[] = {&n_audit, &queue, &counters, &flags, &audit_tol, &oT, &o, (void*)&d_p,
                     (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
-    if ((rc = launch(m, "kx_bk2_fp64_queue", n_audit + cap, stream, args))) return rc;
+    if ((rc = launch(m, "kx_bk2_fp64_queue", n_audit + n, stream, args))) return rc;
   }
   // 4. whole-call fallback, decided on the device: counters[3] = n when the failed audits
   //    exceed max_audit_fail * n_audit, else 0; the plain kernel for all states runs with

@@ -847,13 +847,14 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
         S('            double* __restrict__ Dkm, const double Tref)', 0)
         S('{', 0)
         S('// queue entries [0, n_audit) are audit samples, then counters[0] out-of-hull states;')
-        S('// counters: [0] out-of-hull states queued, [1] failed audits. The grid strides over')
-        S('// the queue, so one launch of any size covers it.')
+        S('// counters: [0] out-of-hull states queued, [1] failed audits. The grid covers the')
+        S('// longest possible queue (every state out of hull); workgroups past the actual end')
+        S('// return before staging the LDS table, after the working ones were dispatched.')
         S('const long long valid = (long long)n_audit + max(counters[0], 0);')
         S(f'if ((long long)blockIdx.x * {block} >= valid) return;   // whole workgroup past the queue')
         S(POOL_MARK)
-        S(f'for (long long slot = (long long)blockIdx.x * {block} + threadIdx.x; slot < valid;')
-        S(f'     slot += (long long)gridDim.x * {block}) {{')
+        S(f'const long long slot = (long long)blockIdx.x * {block} + threadIdx.x;')
+        S('if (slot >= valid) return;')
         S('const long long id = queue[slot];')
         S('const double p = p_state[id];')
     elif pstate:
@@ -954,8 +955,6 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
         S('}')
         S('viscosity[id] = mu;')
     S('conductivity[id] = lambda;')
-    if queue:
-        S('}   // slot')
     S('}', 0)
     return S.text()
 

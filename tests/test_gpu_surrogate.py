@@ -213,8 +213,8 @@ def test_fused_kernel_matches_composed_path(trained, gri, n):
 
 
 def test_out_of_hull_beyond_one_compact_pass(trained, gri):
-    """More out-of-hull states than one compact pass holds (4096 here): the remainder goes
-    through further passes; every state must equal the plain kernel's result."""
+    """Most states out of hull (4700 of 5000; the queue then holds far more than the audit):
+    every one must equal the plain kernel's result."""
     info, t = gri
     kx, sur, _ = trained
     n = 5000
@@ -233,3 +233,27 @@ def test_out_of_hull_beyond_one_compact_pass(trained, gri):
     aud &= ~flagged
     for a, b in zip(got, ref):
         assert torch.equal(a.view(-1, n)[:, aud], b.view(-1, n)[:, aud])
+
+
+def test_enqueue_only_call_matches(trained, gri):
+    """stats=False (kx_sur_transport with no statistics: no host wait, the capturable form)
+    does the same device work: outputs bit-identical to the call that reads its statistics,
+    with out-of-hull states and on a whole-call fallback."""
+    info, t = gri
+    kx, sur, _ = trained
+    N, n = kx.nSpecies(), 3000
+    s, p = states.sample_transport_set(n, info['species'], t['W'], seed=21)
+    s[0, :5] = 3500.0
+    ds = torch.from_numpy(np.ascontiguousarray(s)).to(dev).reshape(-1)
+    dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+    z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
+    for kw in ({}, {'audit_tol': 0.0}):
+        a = [z(n), z(n), z(N * n), z(N * n)]
+        b = [z(n), z(n), z(N * n), z(N * n)]
+        res = sur.infer(n, n, n, dp, ds, *a, **kw)
+        assert sur.infer(n, n, n, dp, ds, *b, stats=False, **kw) is None
+        torch.cuda.synchronize()
+        assert res['out_of_hull'] >= 5
+        assert res['whole_call_fallback'] == ('audit_tol' in kw)
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
