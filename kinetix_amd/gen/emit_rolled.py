@@ -58,14 +58,6 @@ def emit_bk2(t, info, pstate=False):
     S(_table(f'{pre}_diff', pairs), 0)
     S(_table(f'{pre}_C1', t['C1']), 0)
     S(_table(f'{pre}_C2', t['C2']), 0)
-    S('// 1/d for the positive pair fits: hardware reciprocal + two Newton steps (~1 ulp,')
-    S('// no special cases), in place of the IEEE divide sequence')
-    S(f'__device__ __forceinline__ double {pre}_rcp64(double d)', 0)
-    S('{', 0)
-    S('double r = __builtin_amdgcn_rcp(d);')
-    S('r = fma(r, fma(-d, r, 1.0), r);')
-    S('return fma(r, fma(-d, r, 1.0), r);')
-    S('}', 0)
     S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
     if pstate:
         S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
@@ -106,7 +98,7 @@ def emit_bk2(t, info, pstate=False):
     def rd(pidx):
         if rcp_diff:
             return _poly(pre + '_diff', pidx)
-        return f'{pre}_rcp64({_poly(pre + "_diff", pidx)})'
+        return f'kx_rcp({_poly(pre + "_diff", pidx)})'    # positive pair fits (file header)
     for j0 in range(0, N, JB):
         nb = min(JB, N - j0)
         S('{')
