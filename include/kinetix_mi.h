@@ -148,6 +148,10 @@ double kx_ref_mean_molecular_weight(kx_mech_t m);
  * assume, and writes the 16x16 product row-major to d_out (256 floats). */
 int kx_selftest_mfma_layout(float *d_out, void *stream);
 
+/* Diagnostics (no reference counterpart): d_out[i] = the transport kernels' FP64 reciprocal
+ * (v_rcp_f64 + two Newton steps, kinetix_amd/csrc/kx_rcp.h) of d_in[i], i < n. */
+int kx_selftest_rcp(int n, const double *d_in, double *d_out, void *stream);
+
 /* Learned transport surrogate (no reference counterpart): see kinetix_mi_surrogate.h. */
 
 #ifdef __cplusplus

@@ -54,7 +54,8 @@ def generator_digest():
     """SHA-256 over the generator's sources (the code that decides what is generated and how
     it compiles): any change to them invalidates every cache entry."""
     files = sorted(glob.glob(os.path.join(PKG, 'gen', '*.py')))
-    files += [os.path.join(PKG, 'build.py'), os.path.join(PKG, 'csrc', 'kx_mlp_wave.h')]
+    files += [os.path.join(PKG, 'build.py'), os.path.join(PKG, 'csrc', 'kx_mlp_wave.h'),
+              os.path.join(PKG, 'csrc', 'kx_rcp.h')]
     h = hashlib.sha256()
     for f in files:
         h.update(os.path.basename(f).encode() + b'\0')

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/kinetix_mi.h"
+#include "kx_rcp.h"
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -26,5 +27,22 @@ extern "C" int kx_selftest_mfma_layout(float* d_out, void* stream)
 {
   if (!d_out) return KX_ERR_INVALID_ARG;
   hipLaunchKernelGGL(kx_mfma_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, d_out);
+  return hipGetLastError() == hipSuccess ? KX_OK : KX_ERR_HIP;
+}
+
+// kx_rcp (the transport kernels' reciprocal) on n inputs: out[i] = kx_rcp(in[i]).
+__global__ void __launch_bounds__(256) kx_rcp_probe(int n, const double* __restrict__ in,
+                                                    double* __restrict__ out)
+{
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = kx_rcp(in[i]);
+}
+
+extern "C" int kx_selftest_rcp(int n, const double* d_in, double* d_out, void* stream)
+{
+  if (n < 0 || (n > 0 && (!d_in || !d_out))) return KX_ERR_INVALID_ARG;
+  if (n == 0) return KX_OK;
+  hipLaunchKernelGGL(kx_rcp_probe, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                     d_in, d_out);
   return hipGetLastError() == hipSuccess ? KX_OK : KX_ERR_HIP;
 }

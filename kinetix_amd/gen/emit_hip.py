@@ -1009,18 +1009,7 @@ HEADER = '''// GENERATED by kinetix_amd.gen.emit_hip for mechanism "{name}" -- d
 // {n_species} species ({n_active} active), {n_reactions} reactions. Target: gfx950 (CDNA4).
 #include <hip/hip_runtime.h>
 #include <math.h>
-
-// 1/y for finite nonzero y of moderate exponent: v_rcp_f64 refined by two Newton steps, within
-// 1 ulp of the quotient. Five dependent instructions instead of the IEEE division's ~10
-// (scaling, Newton, fmas, fixup) that the transport kernels' ~1,400 pair reciprocals spent.
-__device__ __forceinline__ double kx_rcp(double y)
-{{
-  double r = __builtin_amdgcn_rcp(y);
-  double e = fma(-y, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-y, r, 1.0);
-  return fma(r, e, r);
-}}
+#include "kx_rcp.h"      // kx_rcp: Newton-refined FP64 reciprocal (kinetix_amd/csrc)
 '''
 
 

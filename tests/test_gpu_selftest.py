@@ -43,3 +43,25 @@ def test_mfma_bf16_16x16x32_layout():
     A = np.array([[(i * 7 + k * 3) % 17 - 8 for k in range(32)] for i in range(16)], dtype=np.float64)
     B = np.array([[(k * 5 + j * 11) % 13 - 6 for j in range(16)] for k in range(32)], dtype=np.float64)
     assert np.array_equal(out.cpu().numpy().reshape(16, 16).astype(np.float64), A @ B)
+
+
+def test_newton_reciprocal_within_one_ulp():
+    """kx_rcp, the transport kernels' reciprocal of the positive fit polynomials: within 1 ulp
+    of the exact 1/y over 60 decades of both signs, exact on powers of two."""
+    L = lib()
+    L.kx_selftest_rcp.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(20261202)
+    y = 10.0 ** rng.uniform(-30, 30, 1 << 20)
+    y[: 1 << 18] = rng.uniform(1e-6, 1e3, 1 << 18)            # the fits' own range
+    y[1 << 18: 1 << 19] *= -1.0
+    pw = 2.0 ** np.arange(-60, 61)
+    y = np.concatenate([y, pw, np.nextafter(pw, 0), np.nextafter(pw, np.inf), [1.0, 3.0, 0.1, 7.0]])
+    d_in = torch.from_numpy(y).cuda()
+    d_out = torch.full_like(d_in, np.nan)
+    assert L.kx_selftest_rcp(y.size, C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    r = d_out.cpu().numpy()
+    exact = 1.0 / y                                             # correctly rounded (IEEE)
+    ulp = np.abs(r - exact) / np.spacing(np.abs(exact))
+    assert np.all(np.isfinite(r)) and ulp.max() <= 1.0, ulp.max()
+    assert np.array_equal(r[(1 << 20):(1 << 20) + pw.size], 1.0 / pw)
