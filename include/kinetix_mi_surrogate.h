@@ -76,7 +76,8 @@ size_t kx_sur_workspace_bytes(kx_mech_t m, int n_states, int audit_stride, int w
  *  3. if failures exceed max_audit_fail * audited, the plain kernel for all states (decided
  *     on the device).
  * stats == NULL: every step is enqueued on the stream and the call returns at once (no host
- * wait, no host read, no memset: the form for tight loops). stats != NULL: the call reads the
+ * wait, no host read, no memset: the form for tight loops and stream capture into a graph).
+ * stats != NULL: the call reads the
  * counters and returns after the stream has drained. The device work is the same.
  * Outputs as kx_mixture_avg_transport_pstate (rows k * offset); d_Dkm may be NULL. */
 int kx_sur_transport(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
