@@ -77,8 +77,8 @@ size_t kx_sur_workspace_bytes(kx_mech_t m, int n_states, int audit_stride, int w
  *     on the device).
  * stats == NULL: every step is enqueued on the stream and the call returns at once (no host
  * wait, no host read, no memset: the form for tight loops and stream capture into a graph).
- * stats != NULL: the call reads the
- * counters and returns after the stream has drained. The device work is the same.
+ * stats != NULL: the call reads the counters and returns after the stream has drained. The
+ * device work is the same.
  * Outputs as kx_mixture_avg_transport_pstate (rows k * offset); d_Dkm may be NULL. */
 int kx_sur_transport(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
                      const double *d_state, const float *d_norm, const float *d_ostat,
