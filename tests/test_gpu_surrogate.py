@@ -293,3 +293,21 @@ def test_enqueue_only_call_replays_from_a_graph(trained, gri):
         torch.cuda.synchronize()
         for x, y in zip(got, ref):
             assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize('n,stride,offset', [(1001, 1, 0), (1001, 10, 7), (257, 300, 299), (1, 100, 0)])
+def test_audit_sampling_edges(trained, gri, n, stride, offset):
+    """The plain pass over the audit queue at its edges: every state audited (stride 1), an odd
+    count with an offset, an offset past the last state, a single state. Audited states carry
+    the plain kernel's result exactly; the count matches the sampling rule."""
+    info, t = gri
+    kx, sur, _ = trained
+    s, p = states.sample_transport_set(n, info['species'], t['W'], seed=40 + n)
+    res, got, ref = _run(sur, kx, s, p, audit_stride=stride, audit_offset=offset)
+    aud = np.zeros(n, dtype=bool)
+    aud[offset % stride::stride] = True
+    assert res['audited'] == int(aud.sum())
+    for a, b in zip(got, ref):
+        a_, b_ = a.view(-1, n)[:, aud], b.view(-1, n)[:, aud]
+        assert torch.equal(a_, b_)
+        assert torch.all(torch.isfinite(a.view(-1, n)))
