@@ -52,10 +52,15 @@ def mechanism_dirs():
     return sorted(os.path.dirname(p) for p in glob.glob(os.path.join(PKG, 'mechs', '*', 'kernels.hip')))
 
 
-def build_all():
-    build_host()
-    for d in mechanism_dirs():
-        build_mechanism(d)
+def build_all(jobs=None):
+    """Host library and every mechanism's code object, compiled concurrently (one hipcc
+    process each; the large straight-line mechanisms take minutes apiece)."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as pool:
+        futs = [pool.submit(build_host)] + [pool.submit(build_mechanism, d) for d in mechanism_dirs()]
+        for f in futs:
+            f.result()
 
 
 if __name__ == '__main__':

@@ -4,6 +4,11 @@ the masked MSE, one Adam step), the reference the tests hold kinetix_amd/csrc/kx
 Layer layout and parameter block as in kx_mlp.hip: W1[256][64] b1 W2[256][256] b2 W3 b3
 W4[64][256] b4; GELU in its tanh form; loss = mean over the batch and the first 55 outputs of
 (Y - T)^2.
+
+Pinned by tests/test_mlp_oracle.py, independently of this code: the forward pass and GELU
+against torch's float64 operators, the gradient against fourth-order finite differences of the
+loss (every parameter block) and torch.autograd, GELU's derivative against finite differences,
+and adam_step against the closed form of Adam's first two steps.
 """
 import numpy as np
 
