@@ -27,6 +27,12 @@ PEAK_FP64 = 78.6      # TFLOP/s, MI355X FP64 (vector = matrix)
 PEAK_FP32 = 157.3     # TFLOP/s
 PEAK_HBM = 8000.0     # GB/s
 PEAK_BF16 = 2500.0    # TFLOP/s, dense BF16 MFMA
+# FP32 transcendental rate (v_exp_f32, v_log_f32, ...: quarter rate, 16 of 64 lanes per cycle
+# per SIMD): 256 CUs x 4 SIMDs x 16 lanes / 4 x 2.4 GHz. FP64 exp/log/pow have no hardware
+# instruction (instruction sequences on the FP64 VALU); their rate is reported against the
+# same figure for scale.
+PEAK_TRANSC = 256 * 4 * 16 / 4 * 2.4e9 / 1e12   # T/s
+CPU_SHARE = 16        # host threads one GPU's lease provides
 MLP_FLOPS = 2 * (55 * 256 + 256 * 256 * 2 + 256 * 55)   # useful MACs x 2 per state
 
 
@@ -62,12 +68,17 @@ def parse():
     return ap.parse_args()
 
 
-def host_cores():
+def affinity_cores():
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def host_cores():
+    """Threads for the CPU baseline: the host's affinity set, at most the CPU share that one
+    GPU's lease provides (the rest of the host belongs to other GPUs' jobs)."""
+    return max(1, min(affinity_cores(), CPU_SHARE))
 
 
 def cpu_baseline(state, p, cores, target_s, mech='gri30'):
@@ -99,7 +110,8 @@ def cpu_baseline(state, p, cores, target_s, mech='gri30'):
     run(s[:, :min(n, 1024)])                     # warm
     t0 = time.perf_counter(); run(s); dt = time.perf_counter() - t0
     return {'value': round(n / dt / 1e6, 4), 'unit': 'Mstates/s', 'cores': cores, 'kind': kind,
-            'per_core': round(n / dt / 1e6 / cores, 4),
+            'per_core': round(n / dt / 1e6 / cores, 4), 'affinity_cores': affinity_cores(),
+            'thread_cap': f'min(affinity, {CPU_SHARE}): the CPU share of one GPU',
             'sample': f'{n} of the {n_all} states, BK1 + BK2 FP64, '
                       f'{"reference GRI routines (SERIAL flags)" if kind == "reference" else "oracle port (-O3 -ffast-math)"}, '
                       f'{cores} threads, {dt:.1f} s'}
@@ -258,6 +270,20 @@ def main():
 
     bk2_variants = {'fp64': {'kernel': 'kx_bk2_fp64', 'ms': round(timed(bk2_plain), 4)}}
     p_visc, p_cond, p_rhoD = d_visc.clone(), d_cond.clone(), d_rhoD.clone()
+
+    def bk2_mfma():
+        kx.mixtureAvgTransportProps(n, n, n, 1.0, d_state, d_visc, d_cond, d_rhoD, mfma=True)
+    try:
+        bk2_mfma(); torch.cuda.synchronize()
+        ms_m = timed(bk2_mfma)
+        bk2_variants['fp64_mfma'] = {
+            'kernel': 'kx_bk2_fp64_mfma', 'ms': round(ms_m, 4),
+            'viscosity_max_rel_vs_plain': float((d_visc / p_visc - 1).abs().max()),
+            'conductivity_max_rel_vs_plain': float((d_cond / p_cond - 1).abs().max()),
+            'rhoD_max_rel_vs_plain': float((d_rhoD / p_rhoD - 1).abs().nan_to_num(0.0).max())}
+    except KinetixError as e:              # loop-form mechanisms have no MFMA form
+        bk2_variants['fp64_mfma'] = {'kernel': 'kx_bk2_fp64_mfma', 'absent': str(e)}
+    bk2_plain(); torch.cuda.synchronize()
     sur_err = None
     if sur is not None:
         ms_s = timed(bk2)
@@ -280,8 +306,9 @@ def main():
         bk2_variants['surrogate'] = dict({'kernel': 'kx_sur_transport (kx_sur_infer + 1% audit + fallback)',
                                           'ms': round(ms_s, 4)}, **sur_err)
     for v in bk2_variants.values():
-        v['Mstates_s'] = round(n / v['ms'] / 1e3, 2)
-        v['speedup_vs_fp64'] = round(bk2_variants['fp64']['ms'] / v['ms'], 3)
+        if 'ms' in v:
+            v['Mstates_s'] = round(n / v['ms'] / 1e3, 2)
+            v['speedup_vs_fp64'] = round(bk2_variants['fp64']['ms'] / v['ms'], 3)
 
     # errors of the headline outputs against the FP64 oracle on a sample of states
     rates = d_rates.reshape(N + 1, n)[:, idx].cpu().numpy()
@@ -321,30 +348,55 @@ def main():
            'bk2_conductivity_max_rel': rel(cond, c0), 'bk2_viscosity_max_rel': rel(visc, v0),
            'bk2_rhoD_max_rel': rel(rhoD, r0), 'sample_states': int(len(idx))}
 
+    traffic_path = os.path.join(ROOT, 'profiles', 'traffic.json')
+    traffic = {}
+    if os.path.exists(traffic_path):
+        with open(traffic_path) as fh:
+            traffic = json.load(fh)
+
     def roof(name, ms):
+        """Roofline record of one kernel from its event-timed launch and the generator's
+        per-state counts; `traffic` is the HBM bytes per launch measured with rocprofv3
+        (profiles/traffic.json) for this kernel at this state count, when one exists."""
+        tr = traffic.get(f'{name}@{n}')
         if name == 'kx_sur_infer':
             fl = MLP_FLOPS * n / (ms * 1e-3) / 1e12
             gbs = SUR_BYTES(N) * n / (ms * 1e-3) / 1e9
             return {'bound': 'mfma', 'pipe': 'BF16 MFMA (dense peak 2.5 PFLOP/s)',
                     'achieved': round(fl, 3), 'peak': PEAK_BF16, 'unit': 'TFLOP/s',
-                    'frac': round(fl / PEAK_BF16, 4), 'traffic': None,
+                    'frac': round(fl / PEAK_BF16, 4), 'traffic': tr,
                     'flops_per_state': MLP_FLOPS, 'bytes_per_state': SUR_BYTES(N),
                     'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
                     'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
-        c = counts[name if name in counts else 'kx_bk1_fp64']
+        c = counts[name if name in counts else ('kx_bk2_fp64' if 'bk2' in name else 'kx_bk1_fp64')]
         fl = c['flops'] * n / (ms * 1e-3) / 1e12
         gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
-        peak = PEAK_FP64
-        return {'bound': 'mfma', 'pipe': 'FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)',
-                'achieved': round(fl, 3), 'peak': peak, 'unit': 'TFLOP/s',
-                'frac': round(fl / peak, 4), 'traffic': None,
-                'flops_per_state': c['flops'], 'transcendentals_per_state': c['transcendentals'],
-                'bytes_per_state': c['bytes'],
-                'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
-                'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
+        r = {'bound': 'valu', 'pipe': 'FP64 VALU (MI355X FP64 vector peak 78.6 TFLOP/s)'
+             + (', Wilke sum on FP64 MFMA (same peak)' if name == 'kx_bk2_fp64_mfma' else ''),
+             'achieved': round(fl, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
+             'frac': round(fl / PEAK_FP64, 4), 'traffic': tr,
+             'flops_per_state': c['flops'], 'transcendentals_per_state': c['transcendentals'],
+             'bytes_per_state': c['bytes'],
+             'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
+             'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
+        if tr:
+            r['traffic_over_algorithmic'] = round(tr / (c['bytes'] * n), 3)
+            r['hbm_GBps_measured_traffic'] = round(tr / (ms * 1e-3) / 1e9, 1)
+        if 'bk1' in name:
+            ts = c['transcendentals'] * n / (ms * 1e-3) / 1e12
+            r.update({'transcendentals_per_s_T': round(ts, 3), 'transcendental_peak_T': round(PEAK_TRANSC, 3),
+                      'transcendental_frac': round(ts / PEAK_TRANSC, 4),
+                      'transcendental_note': 'FP32 hardware rate as the scale; FP64 exp/log/pow are '
+                                             'VALU instruction sequences' + (
+                                                 ' (the mixed kernel runs its Arrhenius exponentials '
+                                                 'on v_exp_f32)' if 'mixed' in name else '')})
+        return r
 
     m1, m2 = float(t_bk1.mean()), float(t_bk2.mean())
     kern = {bk1_name: roof(bk1_name, m1)}
+    for v in list(variants.values()) + list(bk2_variants.values()):
+        if 'ms' in v and v['kernel'].startswith('kx_b') and v['kernel'] not in kern:
+            kern[v['kernel']] = roof(v['kernel'], v['ms'])
     if sur is not None:
         # the fused surrogate kernel on its own, event-timed on the launch stream
         from kinetix_amd.surrogate import _p, _s
@@ -364,13 +416,6 @@ def main():
     bk2_kernel = 'kx_sur_infer' if sur is not None else bk2_name
     dom = bk1_name if m1 >= bk2_kernel_ms else bk2_kernel
     roofline = dict(kern[dom]); roofline['kernel'] = dom
-    traffic_path = os.path.join(ROOT, 'profiles', 'traffic.json')
-    if os.path.exists(traffic_path):
-        with open(traffic_path) as fh:
-            tr = json.load(fh)
-        key = f'{dom}@{n}'
-        if key in tr:
-            roofline['traffic'] = tr[key]
 
     out = {
         'metric': METRIC,

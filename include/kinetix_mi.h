@@ -94,15 +94,24 @@ int kx_production_rates(kx_mech_t m, int n_states, int offsetT, int offset, doub
 /* Mixed-precision variant of kx_production_rates (no reference counterpart; the nearest is
  * the reference's "fpmix" kernels, kinetix.cpp:254-281, which run ALL kinetics in FP32):
  * Arrhenius/falloff/third-body in FP32, equilibrium constants, net forward-minus-reverse
- * rates and the per-species sums in FP64, stoichiometric accumulation q.nu on FP32 MFMA.
- * Same layout and units as kx_production_rates. */
+ * rates and the per-species sums in FP64. Same layout and units as kx_production_rates.
+ * Runs the fastest correct mixed kernel, kx_bk1_mixed_valu: the stoichiometric
+ * accumulation q.nu as FP64 FMAs on the vector ALUs (2.1-2.2x the FP64 kernel on MI355X at
+ * GRI-3.0, relative L2 error per species ~1.3e-6 against FP64). */
 int kx_production_rates_mixed(kx_mech_t m, int n_states, int offsetT, int offset,
                               double pressure, const double *d_state, double *d_rates,
                               void *stream);
 
-/* Same precision split as kx_production_rates_mixed, but the stoichiometric accumulation
- * q.nu runs on the FP64 vector ALUs instead of FP32 MFMA (no LDS staging). */
+/* The same kernel as kx_production_rates_mixed, under its explicit name. */
 int kx_production_rates_mixed_valu(kx_mech_t m, int n_states, int offsetT, int offset,
+                                   double pressure, const double *d_state, double *d_rates,
+                                   void *stream);
+
+/* Same precision split, with q.nu accumulated on FP32 MFMA (v_mfma_f32_16x16x4_f32, FP32
+ * accumulation per reaction block, blocks folded in FP64) over the wave's 64 states. Slower
+ * than kx_production_rates_mixed on MI355X (the q values make an LDS round trip); kept for
+ * the MFMA-accumulation form of the mixed kinetics. */
+int kx_production_rates_mixed_mfma(kx_mech_t m, int n_states, int offsetT, int offset,
                                    double pressure, const double *d_state, double *d_rates,
                                    void *stream);
 
@@ -113,6 +122,15 @@ int kx_production_rates_mixed_valu(kx_mech_t m, int n_states, int offsetT, int o
 int kx_mixture_avg_transport(kx_mech_t m, int n_states, int offsetT, int offset,
                              double pressure, const double *d_state, double *d_viscosity,
                              double *d_conductivity, double *d_rhoD, void *stream);
+
+/* Same arguments, outputs and FP64 arithmetic as kx_mixture_avg_transport, with the O(N^2)
+ * Wilke viscosity sum (kinetix_viscosity, mix_transport.py:337-345) computed as three GEMMs
+ * (C1^2, 2 C1 C2, C2^2 against x_j, x_j/eta_j, x_j/eta_j^2) on FP64 MFMA
+ * (v_mfma_f64_16x16x4_f64) over the states of each wavefront. Straight-line mechanisms only
+ * (KX_ERR_NO_KERNEL for the loop-form ones). */
+int kx_mixture_avg_transport_mfma(kx_mech_t m, int n_states, int offsetT, int offset,
+                                  double pressure, const double *d_state, double *d_viscosity,
+                                  double *d_conductivity, double *d_rhoD, void *stream);
 
 /* Per-state-pressure variant (no reference counterpart: the reference takes one scalar
  * p): d_p holds p [Pa] per state; if d_Dkm is non-NULL it also receives the mixture-averaged
@@ -142,11 +160,20 @@ int kx_molar_masses(kx_mech_t m, double *out);
 double kx_ref_pressure(kx_mech_t m);
 double kx_ref_temperature(kx_mech_t m);
 double kx_ref_mean_molecular_weight(kx_mech_t m);
+/* refMassFractions() (kinetix.hpp:100, kinetix.cpp:605): copies the n_species reference mass
+ * fractions given to kx_build into out; needs kx_build. */
+int kx_ref_mass_fractions(kx_mech_t m, double *out);
 
 /* Diagnostics (no reference counterpart): runs one wave of v_mfma_f32_16x16x4_f32 on
  * closed-form 16x4 and 4x16 operands laid out as the generated mixed-precision kernels
  * assume, and writes the 16x16 product row-major to d_out (256 floats). */
 int kx_selftest_mfma_layout(float *d_out, void *stream);
+
+/* Diagnostics (no reference counterpart): one wave of v_mfma_f64_16x16x4_f64 on closed-form
+ * operands (mode 0: every operand entry used; mode 1: D[i][j] = 16 i + j), each lane's four
+ * raw accumulators written to d_out[4 * lane + r] (256 doubles), so that the host can check
+ * the lane layout the FP64-MFMA transport kernel assumes. */
+int kx_selftest_mfma_f64_layout(int mode, double *d_out, void *stream);
 
 /* Diagnostics (no reference counterpart): d_out[i] = the transport kernels' FP64 reciprocal
  * (v_rcp_f64 + two Newton steps, kinetix_amd/csrc/kx_rcp.h) of d_in[i], i < n. */

@@ -46,10 +46,12 @@ def lib():
         L.kx_build.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
         L.kx_finalize.argtypes = [C.c_int]
         for fn in ('kx_production_rates', 'kx_production_rates_mixed',
-                   'kx_production_rates_mixed_valu', 'kx_production_rates_routines'):
+                   'kx_production_rates_mixed_valu', 'kx_production_rates_mixed_mfma',
+                   'kx_production_rates_routines'):
             getattr(L, fn).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
-        for fn in ('kx_mixture_avg_transport', 'kx_mixture_avg_transport_routines'):
+        for fn in ('kx_mixture_avg_transport', 'kx_mixture_avg_transport_routines',
+                   'kx_mixture_avg_transport_mfma'):
             getattr(L, fn).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.kx_mixture_avg_transport_pstate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
@@ -65,6 +67,7 @@ def lib():
         L.kx_species_index.argtypes = [C.c_int, C.c_char_p]
         L.kx_molecular_weights.argtypes = [C.c_int, _dp]
         L.kx_molar_masses.argtypes = [C.c_int, _dp]
+        L.kx_ref_mass_fractions.argtypes = [C.c_int, _dp]
         for fn in ('kx_ref_pressure', 'kx_ref_temperature', 'kx_ref_mean_molecular_weight'):
             getattr(L, fn).argtypes = [C.c_int]
             getattr(L, fn).restype = C.c_double
@@ -156,14 +159,15 @@ class Kinetix:
 
     # kinetix::productionRates
     def productionRates(self, n_states, offsetT, offset, pressure, o_state, o_rates, mixed=False):
-        """mixed: False (FP64), True or 'mfma' (mixed precision, q.nu on FP32 MFMA),
-        'valu' (mixed precision, q.nu on FP64 VALU), 'routines' (FP64 through the generated
-        routine files, as the reference's kernel calls them)."""
+        """mixed: False (FP64); True or 'valu' (mixed precision, q.nu as FP64 FMAs: the fastest
+        mixed kernel, kx_production_rates_mixed); 'mfma' (mixed precision, q.nu on FP32 MFMA,
+        kx_production_rates_mixed_mfma); 'routines' (FP64 through the generated routine
+        files, as the reference's kernel calls them)."""
         need = offsetT + (self.n_species - 1) * offset + n_states if n_states else 0
         _check_tensor('state', o_state, need)
         _check_tensor('rates', o_rates, need)
         fn = {False: lib().kx_production_rates, True: lib().kx_production_rates_mixed,
-              'mfma': lib().kx_production_rates_mixed,
+              'mfma': lib().kx_production_rates_mixed_mfma,
               'valu': lib().kx_production_rates_mixed_valu,
               'routines': lib().kx_production_rates_routines}[mixed]
         _check(fn(self.h, int(n_states), int(offsetT), int(offset), float(pressure), _ptr(o_state),
@@ -174,16 +178,19 @@ class Kinetix:
 
     # kinetix::mixtureAvgTransportProps
     def mixtureAvgTransportProps(self, n_states, offsetT, offset, pressure, o_state, o_viscosity,
-                                 o_thermalConductivity, o_densityDiffCoeffs, routines=False):
+                                 o_thermalConductivity, o_densityDiffCoeffs, routines=False,
+                                 mfma=False):
         """routines=True: through the generated routine files, as the reference's transport
-        kernel calls them (kx_mixture_avg_transport_routines)."""
+        kernel calls them (kx_mixture_avg_transport_routines). mfma=True: the Wilke sum on
+        FP64 MFMA (kx_mixture_avg_transport_mfma), same outputs."""
         need = offsetT + (self.n_species - 1) * offset + n_states if n_states else 0
         _check_tensor('state', o_state, need)
         _check_tensor('viscosity', o_viscosity, n_states)
         _check_tensor('thermalConductivity', o_thermalConductivity, n_states)
         _check_tensor('densityDiffCoeffs', o_densityDiffCoeffs,
                       (self.n_species - 1) * offset + n_states if n_states else 0)
-        fn = lib().kx_mixture_avg_transport_routines if routines else lib().kx_mixture_avg_transport
+        fn = (lib().kx_mixture_avg_transport_routines if routines else
+              lib().kx_mixture_avg_transport_mfma if mfma else lib().kx_mixture_avg_transport)
         _check(fn(self.h, int(n_states), int(offsetT), int(offset), float(pressure), _ptr(o_state),
                   _ptr(o_viscosity), _ptr(o_thermalConductivity), _ptr(o_densityDiffCoeffs),
                   _stream(o_state)))
@@ -243,7 +250,9 @@ class Kinetix:
         return lib().kx_ref_temperature(self.h)
 
     def refMassFractions(self):
-        return self._ref_y.copy()
+        out = np.zeros(self.n_species)
+        _check(lib().kx_ref_mass_fractions(self.h, out.ctypes.data_as(_dp)))
+        return out
 
     def refMeanMolecularWeight(self):
         return lib().kx_ref_mean_molecular_weight(self.h)

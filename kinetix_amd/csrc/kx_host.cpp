@@ -131,7 +131,7 @@ const char* kx_last_error(void) { return g_err.c_str(); }
 static int finish_init(Mech* m, kx_mech_t* out)
 {
   for (const char* k : {"kx_bk1_fp64", "kx_bk1_mixed", "kx_bk1_mixed_valu", "kx_bk2_fp64",
-                        "kx_bk2_fp64_pstate", "kx_thermo_fp64", "kx_sur_features",
+                        "kx_bk2_fp64_pstate", "kx_bk2_fp64_mfma", "kx_thermo_fp64", "kx_sur_features",
                         "kx_sur_targets", "kx_sur_outputs", "kx_sur_infer",
                         "kx_sur_queue_init", "kx_bk2_fp64_queue", "kx_sur_decide",
                         "kx_bk1_routines", "kx_bk2_routines"}) {
@@ -144,7 +144,7 @@ static int finish_init(Mech* m, kx_mech_t* out)
     if (hipFuncGetAttribute(&bound, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, f) != hipSuccess ||
         bound <= 0 || bound > 1024 || bound % 64) {
       (void)hipGetLastError();
-      hipModuleUnload(m->module);
+      (void)hipModuleUnload(m->module);
       delete m;
       return fail(KX_ERR_HIP, std::string("no usable launch bound for ") + k);
     }
@@ -222,7 +222,7 @@ int kx_finalize(kx_mech_t h)
     m = it->second;
     g_mechs.erase(it);
   }
-  if (m->module) { hipSetDevice(m->device); hipModuleUnload(m->module); }
+  if (m->module) { (void)hipSetDevice(m->device); (void)hipModuleUnload(m->module); }
   delete m;
   return KX_OK;
 }
@@ -254,6 +254,13 @@ int kx_production_rates(kx_mech_t h, int n, int offsetT, int offset, double pres
 
 int kx_production_rates_mixed(kx_mech_t h, int n, int offsetT, int offset, double pressure,
                               const double* d_state, double* d_rates, void* stream)
+{
+  return production_rates_impl("kx_bk1_mixed_valu", h, n, offsetT, offset, pressure, d_state,
+                               d_rates, stream);
+}
+
+int kx_production_rates_mixed_mfma(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                                   const double* d_state, double* d_rates, void* stream)
 {
   return production_rates_impl("kx_bk1_mixed", h, n, offsetT, offset, pressure, d_state, d_rates,
                                stream);
@@ -316,6 +323,22 @@ int kx_mixture_avg_transport(kx_mech_t h, int n, int offsetT, int offset, double
 {
   return transport_impl(h, n, offsetT, offset, pressure, nullptr, d_state, d_visc, d_cond, d_rhoD,
                         nullptr, stream);
+}
+
+int kx_mixture_avg_transport_mfma(kx_mech_t h, int n, int offsetT, int offset, double pressure,
+                                  const double* d_state, double* d_visc, double* d_cond,
+                                  double* d_rhoD, void* stream)
+{
+  Mech* m = get(h);
+  int rc = check_layout(m, n, offsetT, offset, d_state, d_rhoD);
+  if (rc) return rc;
+  if (n > 0 && (!d_visc || !d_cond)) return fail(KX_ERR_INVALID_ARG, "null device pointer");
+  long long oT = offsetT, o = offset;
+  double Tref = m->ref_T;
+  const double* s = d_state;
+  const int* n_dev = nullptr;
+  void* args[] = {&n, &oT, &o, &pressure, &s, &d_cond, &d_visc, &d_rhoD, &Tref, &n_dev};
+  return launch(m, "kx_bk2_fp64_mfma", n, stream, args);
 }
 
 int kx_mixture_avg_transport_pstate(kx_mech_t h, int n, int offsetT, int offset,
@@ -452,7 +475,8 @@ size_t kx_sur_workspace_bytes(kx_mech_t h, int n, int audit_stride, int with_dkm
 int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
                      const double* d_state, const float* d_norm, const float* d_ostat,
                      const uint16_t* d_wfrag, const float* d_params, int audit_stride,
-                     int audit_offset, double audit_tol, double max_audit_fail, double* d_visc,
+                     int audit_offset, double audit_tol, double audit_tol_rhoD,
+                     double max_audit_fail, double* d_visc,
                      double* d_cond, double* d_rhoD, double* d_Dkm, void* d_work,
                      size_t work_bytes, kx_sur_stats* stats, void* stream)
 {
@@ -489,14 +513,15 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
                          flags, d_visc, d_cond, d_rhoD, d_Dkm, fbq, counters, stream)))
     return rc;
   // 3. plain kernel over the audit samples + queued out-of-hull states, in place: it reads
-  //    the queued states, compares mu and lambda of the audit entries with the surrogate's,
+  //    the queued states, compares mu, lambda and rho*D of the audit entries with the surrogate's,
   //    and overwrites the outputs with its own. The queue length is read on the device; the
   //    grid covers the longest possible queue (n_audit + n), and the workgroups past the
   //    actual end return at once, after the working ones (the first) were dispatched.
   {
     long long oT = offsetT, o = offset;
     double Tref = m->ref_T;
-    void* args[] = {&n_audit, &queue, &counters, &flags, &audit_tol, &oT, &o, (void*)&d_p,
+    void* args[] = {&n_audit, &queue, &counters, &flags, &audit_tol, &audit_tol_rhoD, &oT, &o,
+                    (void*)&d_p,
                     (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
     if ((rc = launch(m, "kx_bk2_fp64_queue", n_audit + n, stream, args))) return rc;
   }
@@ -573,5 +598,15 @@ int kx_molar_masses(kx_mech_t h, double* out)
 double kx_ref_pressure(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_p : NAN; }
 double kx_ref_temperature(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_T : NAN; }
 double kx_ref_mean_molecular_weight(kx_mech_t h) { Mech* m = get(h); return m ? m->ref_Mbar : NAN; }
+
+int kx_ref_mass_fractions(kx_mech_t h, double* out)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (!m->built) return fail(KX_ERR_NOT_BUILT, "kx_build has not been called");
+  if (!out) return fail(KX_ERR_INVALID_ARG, "null output");
+  std::copy(m->ref_Y.begin(), m->ref_Y.end(), out);
+  return KX_OK;
+}
 
 }  // extern "C"

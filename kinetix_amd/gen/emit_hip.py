@@ -265,7 +265,9 @@ class _RateEmitter:
         elif kind == M.THREE_BODY:
             kf = f'({kinf}) * {self._third(r, sidx, cvar)}'
         elif kind == M.PLOG:
-            raise NotImplementedError('PLOG kernels are not emitted yet')
+            # reached only through the concentration-form options (bk1_wform=False, or
+            # emit_mixed's wform=False); the default w-form emits PLOG (emit_reaction_w)
+            raise NotImplementedError('PLOG needs the w-form emitter (bk1_wform=True)')
         elif not mixed:
             S(f'  const double kinf = {kinf};')
             S(f'  const double Pr = ({k0}) * {self._third(r, sidx, cvar)} / (kinf + {CFLOAT_MIN});')
@@ -819,6 +821,54 @@ def _poly4(c):
             f'{lit(c[4])} * lnT4')
 
 
+def _emit_bk2_prelude(S, t, sid):
+    """T powers, mole fractions x{k} and Mbar of state `sid` (transportProps.okl:12-35)."""
+    N = int(t['n_species'])
+    S(f'const double T = Tref * state[{sid}];')
+    S('const double lnT = log(T);')
+    S('const double rcpT = 1.0 / T;')
+    S('const double sqrT = sqrt(T);')
+    S('const double lnT2 = lnT * lnT;')
+    S('const double lnT3 = lnT * lnT * lnT;')
+    S('const double lnT4 = lnT * lnT * lnT * lnT;')
+    rcpW = t['rcpW']
+    for k in range(N):
+        S(f'double x{k} = fmax(0.0, state[{sid} + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+    S('const double rcpMbar = ' + ' + '.join(f'x{k}' for k in range(N)) + ';')
+    S('const double Mbar = 1.0 / rcpMbar;')
+    for k in range(N):
+        S(f'x{k} *= Mbar;')
+
+
+def _emit_diffusion_pairs(S, t):
+    """Row sums s{k} = sum_{j != k} x_j / D'_kj over the symmetric pair fits, each pair read
+    once and added to both rows (mix_transport.py:414-471)."""
+    N = int(t['n_species'])
+    rcp_diff = bool(int(t.get('rcp_diff', 0)))
+    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj')
+    for k in range(N):
+        S(f'double s{k} = 0.0;')
+    d = t['diff']
+    for k in range(1, N):
+        S('__builtin_amdgcn_sched_barrier(0);')
+        for j in range(k):
+            if rcp_diff:
+                S(f'{{ const double rD = {_poly4(d[k, j])}; s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
+            else:
+                # the fit is positive over its window: a Newton-refined reciprocal
+                S(f'{{ const double rD = kx_rcp({_poly4(d[k, j])}); s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
+
+
+def _emit_conductivity(S, t):
+    """lambda = sqrt(T) * (sum x_k l_k + 1 / sum x_k / l_k) / 2 (mix_transport.py:261-294)."""
+    N = int(t['n_species'])
+    S('// ---- conductivity')
+    S('double sum1 = 0.0, sum2 = 0.0;')
+    for k in range(N):
+        S(f'{{ const double l = {_poly4(t["cond"][k])}; sum1 += x{k} * l; sum2 += x{k} / l; }}')
+    S('const double lambda = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+
+
 def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     """pstate=False: kx_bk2_fp64, scalar p (the reference's transport kernel).
     pstate=True: kx_bk2_fp64_pstate, p [Pa] per state, also writes D_km.
@@ -833,13 +883,12 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     2-3x slower. The queue form is latency-bound (a few 10^4 states): 256 threads, one wave per
     SIMD on every CU, each wave free to use 372 of the 512 registers without a spill."""
     N = int(t['n_species'])
-    rcp_diff = bool(int(t.get('rcp_diff', 0)))
     S = _Src()
     S(f'extern "C" __global__ void __launch_bounds__({block})', 0)
     if queue:
         S('kx_bk2_fp64_queue(const int n_audit,', 0)
         S('            const int* __restrict__ queue, int* __restrict__ counters,', 0)
-        S('            const int* __restrict__ flags, const double tol,', 0)
+        S('            const int* __restrict__ flags, const double tol, const double tol_rhoD,', 0)
         S('            const long long offsetT, const long long offset,', 0)
         S('            const double* __restrict__ p_state,', 0)
         S('            const double* __restrict__ state, double* __restrict__ conductivity,', 0)
@@ -879,41 +928,26 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
         S(f'const long long id = (long long)blockIdx.x * {block} + threadIdx.x;')
         S('if (id >= n_states || (n_dev && id >= *n_dev)) return;')
         S('const double p = ' + ('p_state[id];' if pstate else 'pressure;'))
-    S('const double T = Tref * state[id];')
-    S('const double lnT = log(T);')
-    S('const double rcpT = 1.0 / T;')
-    S('const double sqrT = sqrt(T);')
-    S('const double lnT2 = lnT * lnT;')
-    S('const double lnT3 = lnT * lnT * lnT;')
-    S('const double lnT4 = lnT * lnT * lnT * lnT;')
+    _emit_bk2_prelude(S, t, 'id')
     rcpW, W = t['rcpW'], t['W']
-    for k in range(N):
-        S(f'double x{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
-    S('const double rcpMbar = ' + ' + '.join(f'x{k}' for k in range(N)) + ';')
-    S('const double Mbar = 1.0 / rcpMbar;')
-    for k in range(N):
-        S(f'x{k} *= Mbar;')
     # diffusion first: x and the N row sums are live
-    S('// ---- mixture-averaged diffusion: symmetric pair sums S_k = sum_{j!=k} X_j / D_kj')
-    for k in range(N):
-        S(f'double s{k} = 0.0;')
-    d = t['diff']
-    for k in range(1, N):
-        S('__builtin_amdgcn_sched_barrier(0);')
-        for j in range(k):
-            if rcp_diff:
-                S(f'{{ const double rD = {_poly4(d[k, j])}; s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
-            else:
-                # the fit is positive over its window: a Newton-refined reciprocal
-                S(f'{{ const double rD = kx_rcp({_poly4(d[k, j])}); s{k} += x{j} * rD; s{j} += x{k} * rD; }}')
+    _emit_diffusion_pairs(S, t)
     S('const double TsqrT = T * sqrT;')
     S(f'const double rho = p * {lit(1.0 / float(t["R"]))} * rcpT * Mbar;')
     S('const double pMbar = p * Mbar;')
     def stores(with_dkm):
         for k in range(N):
             S(f'{{ const double D = TsqrT * (Mbar - {lit(W[k])} * x{k}) / (pMbar * s{k});')
-            S(f'  rhoD[{k} * offset + id] = rho * D;' + (f' Dkm[{k} * offset + id] = D; }}' if with_dkm else ' }'))
+            if queue:
+                # audit of rho*D: the surrogate's value is read before it is overwritten (an
+                # unconditional load; the flag masks non-audit entries, no branch per store)
+                S(f'  const double v = rho * D; const double e = fabs(rhoD[{k} * offset + id] / v - 1.0);')
+                S(f'  bad_rhoD |= !(e <= tol_rhoD); rhoD[{k} * offset + id] = v;'
+                  + (f' Dkm[{k} * offset + id] = D; }}' if with_dkm else ' }'))
+            else:
+                S(f'  rhoD[{k} * offset + id] = rho * D;' + (f' Dkm[{k} * offset + id] = D; }}' if with_dkm else ' }'))
     if queue:
+        S('bool bad_rhoD = false;   // max_k |rhoD_k(surrogate) / rhoD_k - 1| > tol_rhoD, or NaN')
         # D_km optional: one uniform branch around the stores (a branch per store splits the
         # kernel into ~10^2 blocks, and the scheduler then spills ~10^4 VGPRs)
         S('if (Dkm) {')
@@ -943,16 +977,12 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     S('const double mu = sqrT * vis;')
     if not queue:
         S('viscosity[id] = mu;')
-    S('// ---- conductivity')
-    S('double sum1 = 0.0, sum2 = 0.0;')
-    for k in range(N):
-        S(f'{{ const double l = {_poly4(t["cond"][k])}; sum1 += x{k} * l; sum2 += x{k} / l; }}')
-    S('const double lambda = sqrT * (0.5 * (sum1 + 1.0 / sum2));')
+    _emit_conductivity(S, t)
     if queue:
         S('if (slot < n_audit && flags[id] == 0) {   // audit of a state the surrogate answered')
         S('  const double ev = fabs(viscosity[id] / mu - 1.0);')
         S('  const double ec = fabs(conductivity[id] / lambda - 1.0);')
-        S('  if (!(ev <= tol) || !(ec <= tol)) atomicAdd(counters + 1, 1);')
+        S('  if (!(ev <= tol) || !(ec <= tol) || bad_rhoD) atomicAdd(counters + 1, 1);')
         S('}')
         S('viscosity[id] = mu;')
     S('conductivity[id] = lambda;')
@@ -1017,12 +1047,12 @@ POOL_MARK = '// @@pool-prologue@@'
 LDS_POOL_MAX_BYTES = 100 * 1024
 
 
-def _with_pool(name, fn, *args, lds=False, block=BLOCK):
+def _with_pool(name, fn, *args, lds=False, block=BLOCK, lds_max=LDS_POOL_MAX_BYTES):
     global _POOL
     _POOL = ConstPool(name, lds=lds)
     try:
         body = fn(*args)
-        if lds and len(_POOL.values) * 8 > LDS_POOL_MAX_BYTES:
+        if lds and len(_POOL.values) * 8 > lds_max:
             # too large for LDS: re-emit reading the table from global memory
             _POOL = ConstPool(name, lds=False)
             body = fn(*args)
@@ -1068,6 +1098,10 @@ def generate(tables, info, opts=None):
                                 lds=o['lds_pool'], block=o['bk2_block']))
         parts.append(_with_pool('kx_bk2_fp64_queue_k', _emit_bk2_fp64, tables, info, True,
                                 256, True, lds=o['lds_pool'], block=256))
+        from . import emit_bk2_mfma as BM
+        # the constant pool goes to LDS only beside the Wilke tables within 160 KB per CU
+        parts.append(_with_pool('kx_bk2_fp64_mfma_k', BM.emit, tables, info, lds=o['lds_pool'],
+                                block=64 * BM.WAVES, lds_max=160 * 1024 - BM.lds_bytes(tables)))
     parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
     from . import emit_mixed
     parts.append(_with_pool('kx_bk1_mixed_k', emit_mixed.emit, tables, info, o['mixed'],

@@ -84,16 +84,12 @@ def mfma_per_state(t):
             'nu_tiles': len(p['tiles']), 'nu_tiles_dense': len(p['chunks']) * p['nmb']}
 
 
-DEFAULT_OPTS = {'sync': 'wave', 'nu_in_lds': True, 'kstep_outer': True, 'wform': True}
+DEFAULT_OPTS = {'sync': 'wave', 'kstep_outer': True, 'wform': True}
 
 
 def emit(t, info, opts=None):
     o = dict(DEFAULT_OPTS)
     o.update(opts or {})
-    if not o['nu_in_lds']:
-        # MFMA B operand read straight from global memory: this form produced wrong results
-        # on gfx950 (the LDS form, same arithmetic, is correct); disabled until explained
-        raise NotImplementedError('nu_in_lds=False is disabled (known-wrong results)')
     N = int(t['n_species'])
     p = plan(t)
     em = E._RateEmitter(t, info)
@@ -105,7 +101,6 @@ def emit(t, info, opts=None):
                 '__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
     else:
         SYNC = '__syncthreads();'
-    nu_ref = 'nu_s' if o['nu_in_lds'] else 'kx_bk1_mixed_nu'
     S = E._Src()
     tab = p['table'].reshape(-1)
     nt = len(p['tiles'])
@@ -127,11 +122,11 @@ def emit(t, info, opts=None):
     S(E.POOL_MARK)
     S(f'__shared__ __attribute__((aligned(16))) float qs[{WAVES}][64][{QSTRIDE}];')
     S(f'__shared__ __attribute__((aligned(16))) float wt[{WAVES}][64][{WSTRIDE}];')
-    if o['nu_in_lds']:
-        S(f'__shared__ __attribute__((aligned(16))) float nu_s[{max(nt, 1) * 256}];')
-        S(f'for (int i = threadIdx.x; i < {max(nt, 1) * 64}; i += {64 * WAVES})')
-        S('  *(kx_floatx4*)&nu_s[4 * i] = *(const kx_floatx4*)&kx_bk1_mixed_nu[4 * i];')
-        S('__syncthreads();')
+    # the nu tiles (A operands) are copied once per workgroup into LDS and read from there
+    S(f'__shared__ __attribute__((aligned(16))) float nu_s[{max(nt, 1) * 256}];')
+    S(f'for (int i = threadIdx.x; i < {max(nt, 1) * 64}; i += {64 * WAVES})')
+    S('  *(kx_floatx4*)&nu_s[4 * i] = *(const kx_floatx4*)&kx_bk1_mixed_nu[4 * i];')
+    S('__syncthreads();')
     S('const int lane = threadIdx.x & 63;')
     S('const int wv = threadIdx.x >> 6;')
     S(f'const long long id = (long long)blockIdx.x * {64 * WAVES} + threadIdx.x;')
@@ -185,7 +180,7 @@ def emit(t, info, opts=None):
             S(SYNC)
             S('{')
             for (ti, m) in tiles_by_chunk.get(c, []):
-                S(f'  const kx_floatx4 a{m} = *(const kx_floatx4*)&{nu_ref}[({ti} * 64 + lane) * 4];')
+                S(f'  const kx_floatx4 a{m} = *(const kx_floatx4*)&nu_s[({ti} * 64 + lane) * 4];')
             if o['kstep_outer']:
                 # consecutive MFMAs write different accumulators (no back-to-back dependency)
                 for nb in range(4):

@@ -13,8 +13,9 @@ all inside one C-ABI call, kx_sur_transport:
     -> plain kernel;
   * audit: every `audit_stride`-th state (1 % by default) is also computed by the plain
     kernel and keeps the plain result; one whose mu or lambda differs by more than
-    `audit_tol` counts as a failure; if more than `max_audit_fail` of the audited states
-    fail, the whole call falls back to the plain kernel.
+    `audit_tol`, or whose rho*D_k differs by more than `audit_tol_rhoD` for any k, counts as
+    a failure; if more than `max_audit_fail` of the audited states fail, the whole call
+    falls back to the plain kernel.
 """
 from __future__ import annotations
 
@@ -27,10 +28,12 @@ import time
 import numpy as np
 
 from . import states as st
-from .api import Kinetix, KinetixError, _check, lib
+from .api import Kinetix, KinetixError, _check, _check_tensor, lib
 
 IN = 64
 VALID = 55
+AUDIT_TOL = 1e-3          # mu, lambda: the accuracy bound the surrogate is built to
+AUDIT_TOL_RHOD = 1e-2     # rho*D_k, max over k
 
 
 class SurStats(C.Structure):
@@ -48,7 +51,7 @@ def _bind():
     L.kx_sur_workspace_bytes.argtypes = [ci, ci, ci, ci]
     L.kx_sur_workspace_bytes.restype = C.c_size_t
     L.kx_sur_transport.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, C.c_double,
-                                   C.c_double, vp, vp, vp, vp, vp, C.c_size_t,
+                                   C.c_double, C.c_double, vp, vp, vp, vp, vp, C.c_size_t,
                                    C.POINTER(SurStats), vp]
     L.kx_mlp_forward.argtypes = [vp, ci, vp, vp, vp]
     L.kx_mlp_backward.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp, vp]
@@ -237,20 +240,35 @@ class Surrogate:
 
     # ---------------------------------------------------------------- inference
     def infer(self, n, offsetT, offset, d_p, d_state, d_visc, d_cond, d_rhoD, d_Dkm=None,
-              audit_stride=100, audit_tol=1e-3, max_audit_fail=0.01, audit_offset=0, stats=True):
+              audit_stride=100, audit_tol=AUDIT_TOL, audit_tol_rhoD=AUDIT_TOL_RHOD,
+              max_audit_fail=0.01, audit_offset=0, stats=True):
         """Audited surrogate transport for n states (FP64 outputs in the plain kernel's layout),
         one call of kx_sur_transport. Returns the fallback statistics; stats=False enqueues
-        the call on the current stream without waiting and returns None (same device work)."""
+        the call on the current stream without waiting and returns None (same device work).
+
+        The workspace is kept per size and never freed while this object lives, so a graph
+        that captured a call keeps a valid workspace for its replays."""
         import torch
+        N = self.N
+        if n:
+            _check_tensor('state', d_state, offsetT + (N - 1) * offset + n)
+            for nm, t_ in (('p', d_p), ('viscosity', d_visc), ('conductivity', d_cond)):
+                _check_tensor(nm, t_, n)
+            _check_tensor('rhoD', d_rhoD, (N - 1) * offset + n)
+            if d_Dkm is not None:
+                _check_tensor('Dkm', d_Dkm, (N - 1) * offset + n)
         need = int(self.L.kx_sur_workspace_bytes(self.kx.h, n, audit_stride, d_Dkm is not None))
-        if getattr(self, '_work', None) is None or self._work.numel() < need:
-            self._work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.dev)
+        works = self.__dict__.setdefault('_works', [])
+        work = next((w for w in works if w.numel() >= need), None)
+        if work is None:
+            work = torch.empty(max(need, 256), dtype=torch.uint8, device=self.dev)
+            works.append(work)
         st = SurStats()
         _check(self.L.kx_sur_transport(self.kx.h, n, offsetT, offset, _p(d_p), _p(d_state),
                                        _p(self.norm), _p(self.ostat), _p(self.wbf16), _p(self.params),
-                                       audit_stride, audit_offset, audit_tol, max_audit_fail,
-                                       _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm), _p(self._work),
-                                       self._work.numel(), C.byref(st) if stats else None, _s()))
+                                       audit_stride, audit_offset, audit_tol, audit_tol_rhoD,
+                                       max_audit_fail, _p(d_visc), _p(d_cond), _p(d_rhoD), _p(d_Dkm),
+                                       _p(work), work.numel(), C.byref(st) if stats else None, _s()))
         if not stats:
             return None
         return {'n': st.n_states, 'out_of_hull': st.out_of_hull, 'audited': st.audited,

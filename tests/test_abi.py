@@ -30,7 +30,8 @@ def L():
 def test_headers_declare_the_boundary():
     names = declared()
     for must in ('kx_init', 'kx_build', 'kx_finalize', 'kx_production_rates',
-                 'kx_production_rates_mixed', 'kx_mixture_avg_transport',
+                 'kx_production_rates_mixed', 'kx_production_rates_mixed_mfma',
+                 'kx_ref_mass_fractions', 'kx_mixture_avg_transport',
                  'kx_mixture_avg_transport_pstate', 'kx_thermodynamic_props', 'kx_n_species',
                  'kx_species_name', 'kx_sur_transport', 'kx_sur_workspace_bytes',
                  'kx_mlp_forward', 'kx_mlp_backward', 'kx_mlp_adam', 'kx_mlp_infer_bf16'):
@@ -55,6 +56,8 @@ def test_unknown_handle_is_an_error_not_a_launch(L):
     assert rc == -2                                                    # KX_ERR_NOT_INITIALIZED
     assert b'handle' in L.kx_last_error()
     assert L.kx_n_species(987654) == -2
+    out = (C.c_double * 4)()
+    assert L.kx_ref_mass_fractions(987654, out) == -2
     rc = L.kx_mlp_forward(None, 64, None, None, None)
     assert rc == -1                                                    # KX_ERR_INVALID_ARG
     rc = L.kx_mlp_forward(C.c_void_p(16), 65, C.c_void_p(16), C.c_void_p(16), None)

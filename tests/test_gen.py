@@ -31,7 +31,8 @@ def test_emitted_kernels_and_counts():
     src = open(os.path.join(MDIR, 'kernels.hip')).read()
     for k in ('kx_bk1_fp64', 'kx_bk1_mixed', 'kx_bk1_mixed_valu', 'kx_bk2_fp64',
               'kx_bk2_fp64_pstate', 'kx_thermo_fp64', 'kx_sur_features', 'kx_sur_targets',
-              'kx_sur_outputs', 'kx_sur_infer', 'kx_sur_queue_init', 'kx_bk2_fp64_queue'):
+              'kx_sur_outputs', 'kx_sur_infer', 'kx_sur_queue_init', 'kx_bk2_fp64_queue',
+              'kx_bk2_fp64_mfma'):
         assert f'\n{k}(' in src, k
     counts = json.load(open(os.path.join(MDIR, 'counts.json')))
     assert counts['kx_bk1_fp64']['bytes'] == 16 * 54
@@ -108,6 +109,130 @@ def test_fragment_chain_keeps_activations_in_registers():
             nxt = np.concatenate([Z[2 * kb, lane], Z[2 * kb + 1, lane]])
             for e in range(8):
                 assert nxt[e] == pytest.approx(ref[lane % 16, _kphi(kb, lane // 16, e)])
+
+
+def _mfma_f32_16x16x4(a, b, c):
+    """v_mfma_f32_16x16x4_f32 on the lane layout kx_selftest_mfma_layout checks on the device:
+    a, b: [64 lanes] (one value each), c: [64][4]. A[i][k] = a[i + 16k], B[k][j] = b[j + 16k],
+    D[i][j] -> c[j + 16 (i // 4)][i % 4]."""
+    A = np.array([[a[i + 16 * k] for k in range(4)] for i in range(16)])
+    B = np.array([[b[j + 16 * k] for j in range(16)] for k in range(4)])
+    D = A @ B
+    d = c.copy()
+    for i in range(16):
+        for j in range(16):
+            d[j + 16 * (i // 4)][i % 4] += D[i, j]
+    return d
+
+
+def test_mixed_mfma_nu_fragments_accumulate_q_dot_nu(gri):
+    """The q.nu accumulation of kx_bk1_mixed, emulated on the CPU exactly as the kernel
+    addresses its operands: the A fragment of nu tile ti is read at flat float index
+    (ti * 64 + lane) * 4 + s of the nu table (the same index into the LDS copy and into the
+    global table), the B fragment from the q staging rows (state nb*16 + lane%16, reaction
+    slot 4*(lane//16) + s), four K-steps per 16-reaction chunk, the block accumulators
+    transposed to one row per state and summed. The result must be nu^T q for every species
+    and state of a 64-state wave."""
+    from kinetix_amd.gen import emit_mixed
+    _, t = gri
+    p = emit_mixed.plan(t)
+    net = (t['nu_r'] - t['nu_f']).astype(np.float64)
+    R, N = net.shape
+    flat = p['table'].reshape(-1).astype(np.float64)
+    tiles_by_chunk = {}
+    for ti, (c, m) in enumerate(p['tiles']):
+        tiles_by_chunk.setdefault(c, []).append((ti, m))
+    rng = np.random.default_rng(0)
+    q = rng.uniform(-1, 1, size=(R, 64)) * 2.0 ** rng.integers(-20, 20, size=(R, 64))
+    wd = np.zeros((64, 16 * p['nmb']))
+    for blk in p['blocks']:
+        acc = np.zeros((p['nmb'], 4, 64, 4))
+        for c in blk:
+            ch = p['chunks'][c]
+            qs = np.zeros((64, emit_mixed.CHUNK))               # one row per state
+            qs[:, :len(ch)] = q[ch].T
+            for nb in range(4):
+                for s in range(4):
+                    b = np.array([qs[nb * 16 + (lane & 15), 4 * (lane >> 4) + s] for lane in range(64)])
+                    for ti, m in tiles_by_chunk.get(c, []):
+                        a = np.array([flat[(ti * 64 + lane) * 4 + s] for lane in range(64)])
+                        acc[m, nb] = _mfma_f32_16x16x4(a, b, acc[m, nb])
+        for m in range(p['nmb']):
+            for nb in range(4):
+                for lane in range(64):
+                    wd[nb * 16 + (lane & 15), m * 16 + 4 * (lane >> 4):m * 16 + 4 * (lane >> 4) + 4] += acc[m, nb, lane]
+    ref = net.T @ q                                              # [species][state]
+    for k in range(N):
+        if k in p['pos']:
+            np.testing.assert_allclose(wd[:, p['pos'][k]], ref[k], rtol=1e-12, atol=1e-12 * np.abs(q).max())
+        else:
+            assert np.all(ref[k] == 0)                           # inert: never a row
+
+
+def _mfma_f64_16x16x4(a, b, c):
+    """v_mfma_f64_16x16x4_f64 on the lane layout kx_selftest_mfma_f64_layout checks on the
+    device: A[i][k] = a[i + 16k], B[k][j] = b[j + 16k], D[i][j] -> c[j + 16 (i % 4)][i // 4]."""
+    A = np.array([[a[i + 16 * k] for k in range(4)] for i in range(16)])
+    B = np.array([[b[j + 16 * k] for j in range(16)] for k in range(4)])
+    D = A @ B
+    d = c.copy()
+    for i in range(16):
+        for j in range(16):
+            d[j + 16 * (i % 4)][i // 4] += D[i, j]
+    return d
+
+
+def test_wilke_mfma_fragments_reproduce_the_viscosity(gri, oracle):
+    """kx_bk2_fp64_mfma's Wilke sum, emulated on the CPU exactly as the kernel lays out its
+    operands: A fragments from emit_bk2_mfma.a_fragments (lane l of (ks, m, g) at flat index
+    ((ks * M + m) * 3 + g) * 64 + l), B operands (x_j, x_j/eta_j, x_j/eta_j^2 of species
+    4 ks + l // 16, state 16 sb + l % 16), the epilogue over the result layout (element r of
+    lane l: species 16 m + 4 r + l // 16) and the four-lane-group sum. The viscosity of 64 seeded states must match the oracle's."""
+    from kinetix_amd import states
+    from kinetix_amd.gen import emit_bk2_mfma as BM
+    info, t = gri
+    N = int(t['n_species'])
+    M, KS = BM.tiles(N)
+    flat = BM.a_fragments(t).reshape(-1)
+    sp = BM.species_table(t)
+    s, _ = states.sample(64, info['species'], t['W'], T_range=(300.0, 2500.0), seed=5,
+                         p_range_atm=(0.5, 40.0))
+    T = s[0]
+    lt = np.log(T)
+    P = np.stack([lt ** i for i in range(5)])                   # (5, 64)
+    x = np.maximum(s[1:], 0.0) * sp[:, 5:6]
+    x = x / x.sum(0)                                             # mole fractions (N, 64)
+    eta = sp[:, :5] @ P                                          # (N, 64)
+    vis = np.zeros(64)
+    for sb in range(4):
+        acc = np.zeros((M, 3, 64, 4))
+        for ks in range(KS):
+            u = np.zeros(64); v = np.zeros(64); w = np.zeros(64)
+            for lane in range(64):
+                j, st = 4 * ks + lane // 16, 16 * sb + lane % 16
+                if j < N:
+                    u[lane] = x[j, st]
+                    v[lane] = u[lane] / eta[j, st]
+                    w[lane] = v[lane] / eta[j, st]
+            for m in range(M):
+                for g, b in enumerate((u, v, w)):
+                    off = ((ks * M + m) * 3 + g) * 64
+                    acc[m, g] = _mfma_f64_16x16x4(flat[off:off + 64], b, acc[m, g])
+        part = np.zeros(64)
+        for lane in range(64):
+            st = 16 * sb + lane % 16
+            for m in range(M):
+                for r in range(4):
+                    k = 16 * m + 4 * r + lane // 16
+                    if k < N:
+                        e = eta[k, st]
+                        Sk = acc[m, 0, lane, r] + e * (acc[m, 1, lane, r] + e * acc[m, 2, lane, r])
+                        part[lane] += x[k, st] * e * e / Sk
+        tot = part[:16] + part[16:32] + part[32:48] + part[48:]
+        vis[16 * sb:16 * sb + 16] = tot
+    mu = np.sqrt(T) * vis
+    _, v0, _, _ = oracle.transport(s, 1.0)
+    assert np.max(np.abs(mu / v0 - 1)) < 1e-13
 
 
 def test_liveness_order(gri):

@@ -43,6 +43,12 @@ def l2_per_species(x, ref):
 ACCUM = ['mfma', 'valu']
 
 
+def test_mixed_entry_point_runs_the_fastest_kernel(kx, golden):
+    """kx_production_rates_mixed (mixed=True) is kx_bk1_mixed_valu: bit-identical outputs."""
+    s, p = golden['state_bk1'], float(golden['p_bk1'])
+    assert np.array_equal(run(kx, s, p, mixed=True), run(kx, s, p, mixed='valu'))
+
+
 @pytest.mark.parametrize('accum', ACCUM)
 def test_mixed_golden_states(kx, oracle, golden, accum):
     s, p = golden['state_bk1'], float(golden['p_bk1'])

@@ -22,6 +22,34 @@ def test_mfma_f32_16x16x4_layout():
     assert np.array_equal(out.cpu().numpy().reshape(16, 16).astype(np.float64), A @ B)
 
 
+def _f64_probe(mode):
+    L = lib()
+    L.kx_selftest_mfma_f64_layout.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+    out = torch.full((256,), np.nan, dtype=torch.float64, device='cuda')
+    assert L.kx_selftest_mfma_f64_layout(mode, C.c_void_p(out.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(64, 4)
+
+
+def _raw(D):
+    """D (16x16) in the FP64 result layout: lane l, element r = D[4 r + l // 16][l % 16]."""
+    return np.array([[D[4 * r + l // 16, l % 16] for r in range(4)] for l in range(64)])
+
+
+def test_mfma_f64_16x16x4_result_layout():
+    """Every result entry names its own (row, column): D[i][j] = 16 i + j."""
+    got = _f64_probe(1)
+    D = np.array([[16.0 * i + j for j in range(16)] for i in range(16)])
+    assert np.array_equal(got, _raw(D)), got
+
+
+def test_mfma_f64_16x16x4_operand_layout():
+    got = _f64_probe(0)
+    A = np.array([[i + 16 * k + 1 for k in range(4)] for i in range(16)], dtype=np.float64)
+    B = np.array([[(k + 1) * 100 + j for j in range(16)] for k in range(4)], dtype=np.float64)
+    assert np.array_equal(got, _raw(A @ B))
+
+
 def test_mfma_bf16_16x16x16_layout():
     L = lib()
     L.kx_selftest_mfma_bf16_layout.argtypes = [C.c_void_p, C.c_void_p]

@@ -14,6 +14,10 @@ from oracle import mlp_oracle as mo
 
 pytestmark = pytest.mark.gpu
 dev = torch.device('cuda')
+# the rho*D audit tolerance for the smoke-sized model trained below (131,072 states, 4
+# epochs, for speed): its rho*D errors are ~10x the shipped model's, whose own tolerance
+# (kinetix_amd.surrogate.AUDIT_TOL_RHOD) the shipped-model tests use
+SMOKE_TOL_RHOD = 0.2
 
 
 def t32(a):
@@ -112,6 +116,7 @@ def _run(sur, kx, s, p, **kw):
     dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
     z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
     v, c, r, d = z(n), z(n), z(N * n), z(N * n)
+    kw.setdefault('audit_tol_rhoD', SMOKE_TOL_RHOD)
     info = sur.infer(n, n, n, dp, ds, v, c, r, d, **kw)
     pv, pc, pr, pd = sur.plain(n, n, n, dp, ds)
     torch.cuda.synchronize()
@@ -167,6 +172,21 @@ def test_audit_failure_falls_back_whole_call(trained, gri):
     assert res['whole_call_fallback'] and res['fallback_fraction'] == 1.0
     for a, b in zip(got, ref):
         assert torch.equal(a, b)
+
+
+def test_rhoD_audit_failure_falls_back_whole_call(trained, gri):
+    """A rho*D mismatch alone (mu and lambda tolerances wide open) fails the audit samples and
+    routes the whole call to the plain kernel."""
+    info, t = gri
+    kx, sur, _ = trained
+    s, p = states.sample_transport_set(3000, info['species'], t['W'], seed=6)
+    res, got, ref = _run(sur, kx, s, p, audit_tol=1.0, audit_tol_rhoD=0.0)
+    assert res['audit_failed'] == res['audited'] > 0
+    assert res['whole_call_fallback'] and res['fallback_fraction'] == 1.0
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+    res, _, _ = _run(sur, kx, s, p, audit_tol=1.0, audit_tol_rhoD=1.0)
+    assert res['audit_failed'] == 0 and not res['whole_call_fallback']
 
 
 def test_corrupted_weights_are_caught(trained, gri):
@@ -248,8 +268,10 @@ def test_enqueue_only_call_matches(trained, gri):
     dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
     z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
     for kw in ({}, {'audit_tol': 0.0}):
+        kw = dict(kw)
         a = [z(n), z(n), z(N * n), z(N * n)]
         b = [z(n), z(n), z(N * n), z(N * n)]
+        kw['audit_tol_rhoD'] = SMOKE_TOL_RHOD
         res = sur.infer(n, n, n, dp, ds, *a, **kw)
         assert sur.infer(n, n, n, dp, ds, *b, stats=False, **kw) is None
         torch.cuda.synchronize()
