@@ -94,6 +94,28 @@ int kx_sur_transport(kx_mech_t m, int n_states, int offsetT, int offset, const d
                      double *d_rhoD, double *d_Dkm, void *d_workspace, size_t workspace_bytes,
                      kx_sur_stats *stats, void *stream);
 
+/* Size of the device scratch array behind kx_sur_transport_noalloc (64 MiB per GPU, held
+ * from the first kx_init on that GPU until the process ends). kx_sur_workspace_bytes(n,
+ * audit_stride) must not exceed it: up to 5,592,384 states at audit_stride 1 and 8,346,816
+ * at the default stride of 100. */
+#define KX_SUR_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). kx_sur_transport
+ * with the workspace taken from a library-owned device array (KX_SUR_SCRATCH_BYTES per GPU)
+ * instead of an argument; it calls no hipMalloc/hipFree and allocates nothing. Calls beyond
+ * the scratch size return KX_ERR_TOO_LARGE. Every call on one GPU shares that GPU's scratch,
+ * which keeps the last call's contents (each call initialises what it reads): calls that
+ * overlap on one GPU (other streams, other threads) race -- serialise them, or use
+ * kx_sur_transport with a workspace each. Same arguments and results otherwise. */
+int kx_sur_transport_noalloc(kx_mech_t m, int n_states, int offsetT, int offset, const double *d_p,
+                             const double *d_state, const float *d_norm, const float *d_ostat,
+                             const uint16_t *d_weights_bf16, const float *d_params,
+                             int audit_stride, int audit_offset, double audit_tol,
+                             double audit_tol_rhoD, double max_audit_fail, double *d_viscosity,
+                             double *d_conductivity, double *d_rhoD, double *d_Dkm,
+                             kx_sur_stats *stats, void *stream);
+
 /* ---- mechanism-independent MLP kernels ---- */
 
 int kx_mlp_n_params(void);          /* FP32 parameter block length */

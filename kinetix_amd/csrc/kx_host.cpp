@@ -59,6 +59,30 @@ std::mutex g_mu;
 std::map<int, Mech*> g_mechs;
 int g_next = 1;
 
+// ---- device scratch for kx_sur_transport_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends. Host code must not pass `scratch` itself to a kernel (in
+// host code the name is a stand-in, not a device address): kx_init resolves each device's
+// address once with hipGetSymbolAddress, with that device current, and the entry point uses
+// only that address.
+constexpr size_t kScratchBytes = KX_SUR_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// with `device` current
+int scratch_setup(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(KX_ERR_INVALID_ARG, "device id out of range");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_scratch_ptr[device]) return KX_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(KX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return KX_OK;
+}
+
 Mech* get(kx_mech_t h)
 {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -172,6 +196,7 @@ int kx_init(const char* mech_dir, int device_id, kx_mech_t* out)
     return fail(e == hipErrorFileNotFound ? KX_ERR_IO : KX_ERR_HIP,
                 std::string("hipModuleLoad(") + mech_dir + "/kernels.hsaco): " + hipGetErrorString(e));
   }
+  if ((rc = scratch_setup(device_id))) { (void)hipModuleUnload(m->module); delete m; return rc; }
   return finish_init(m, out);
 }
 
@@ -192,6 +217,7 @@ int kx_init_image(const char* mech_txt, size_t mech_txt_bytes, const void* code_
     delete m;
     return fail(KX_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
   }
+  if ((rc = scratch_setup(device_id))) { (void)hipModuleUnload(m->module); delete m; return rc; }
   return finish_init(m, out);
 }
 
@@ -549,6 +575,33 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
     *stats = st;
   }
   return KX_OK;
+}
+
+int kx_sur_transport_noalloc(kx_mech_t h, int n, int offsetT, int offset, const double* d_p,
+                             const double* d_state, const float* d_norm, const float* d_ostat,
+                             const uint16_t* d_wfrag, const float* d_params, int audit_stride,
+                             int audit_offset, double audit_tol, double audit_tol_rhoD,
+                             double max_audit_fail, double* d_visc, double* d_cond,
+                             double* d_rhoD, double* d_Dkm, kx_sur_stats* stats, void* stream)
+{
+  Mech* m = get(h);
+  if (!m) return fail(KX_ERR_NOT_INITIALIZED, "unknown mechanism handle");
+  if (n < 0) return fail(KX_ERR_INVALID_ARG, "n_states < 0");
+  const size_t need = sur_plan(n, audit_stride, audit_offset).total;
+  if (need > kScratchBytes)
+    return fail(KX_ERR_TOO_LARGE, "kx_sur_transport_noalloc: " + std::to_string(n) +
+                " states at audit stride " + std::to_string(audit_stride) + " need " +
+                std::to_string(need) + " bytes of scratch, more than KX_SUR_SCRATCH_BYTES (" +
+                std::to_string(kScratchBytes) + "); use kx_sur_transport with a workspace");
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ptr = (m->device >= 0 && m->device < kMaxDevices) ? g_scratch_ptr[m->device] : nullptr;
+  }
+  if (!ptr) return fail(KX_ERR_NOT_INITIALIZED, "no scratch address for this device");
+  return kx_sur_transport(h, n, offsetT, offset, d_p, d_state, d_norm, d_ostat, d_wfrag, d_params,
+                          audit_stride, audit_offset, audit_tol, audit_tol_rhoD, max_audit_fail,
+                          d_visc, d_cond, d_rhoD, d_Dkm, ptr, kScratchBytes, stats, stream);
 }
 
 int kx_n_species(kx_mech_t h) { Mech* m = get(h); return m ? m->n_species : KX_ERR_NOT_INITIALIZED; }
