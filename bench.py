@@ -175,6 +175,7 @@ def main():
         sur = Surrogate(kx, path)
     sur_stats = []
     sur_calls = [0]
+    from kinetix_amd.api import KinetixError
 
     def bk1(mode=None):
         kx.productionRates(n, n, n, 1.0, d_state, d_rates, mixed=bk1_mode if mode is None else mode)
@@ -200,12 +201,21 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        bk1(); torch.cuda.synchronize()
-        bk2_plain(); torch.cuda.synchronize()
+        launched = []
+        for vname, (kname, mode) in BK1.items():
+            try:
+                bk1(mode); torch.cuda.synchronize(); launched.append(kname)
+            except KinetixError:
+                pass
+        bk2_plain(); torch.cuda.synchronize(); launched.append('kx_bk2_fp64')
+        try:
+            kx.mixtureAvgTransportProps(n, n, n, 1.0, d_state, d_visc, d_cond, d_rhoD, mfma=True)
+            torch.cuda.synchronize(); launched.append('kx_bk2_fp64_mfma')
+        except KinetixError:
+            pass
         if sur is not None:
-            bk2(); torch.cuda.synchronize()
-        print(f'profile-only: copy {src.numel() * 8} B, {bk1_name}, kx_bk2_fp64'
-              f'{" and the surrogate" if sur else ""} over {n} states')
+            bk2(); torch.cuda.synchronize(); launched.append('the surrogate')
+        print(f'profile-only: copy {src.numel() * 8} B, {", ".join(launched)} over {n} states')
         return 0
 
     for _ in range(a.warmup):
@@ -230,7 +240,6 @@ def main():
     idx = np.random.default_rng(7).choice(n, min(a.err_sample, n), replace=False)
     ref, scale = orc.production_rates(st[:, idx], P_ATM, scale=True, threads=cores)
     variants = {}
-    from kinetix_amd.api import KinetixError
     for vname, (kname, mode) in BK1.items():
         try:
             bk1(mode); torch.cuda.synchronize()
