@@ -685,7 +685,10 @@ def _emit_bk1_live(t, info, variant='fp64', arrh32=False, order=None, waves=0):
     LDS table read at a per-lane offset (T > T_mid selects the piece without a branch).
     waves > 0 asks the compiler for that many waves per SIMD (register budget 512/waves):
     the mixed kernel runs 8 % faster at 2 (some VGPRs go to scratch, latency hides behind
-    the second wave); the FP64 kernel, with more live FP64 state, is faster at 1."""
+    the second wave); the FP64 kernel, with more live FP64 state, is faster at 1. Tried for
+    the mixed kernel's 128 spilled VGPRs, not kept: reading Y_k again at each species' first
+    use (through an opaque pointer) instead of keeping its concentration live, and scheduling
+    barriers every 1-16 reactions -- 126-166 spilled VGPRs in every combination."""
     from . import liveness
     N = int(t['n_species'])
     em = _RateEmitter(t, info)
@@ -1063,7 +1066,7 @@ def _with_pool(name, fn, *args, lds=False, block=BLOCK, lds_max=LDS_POOL_MAX_BYT
 
 DEFAULT_OPTS = {'bk1_sched_window': 0, 'bk1_wform': True, 'mixed': None, 'lds_pool': True,
                 'mixed_lds_pool': False, 'mixed_arrh32': True, 'bk1_live': True,
-                'mixed_waves': 2, 'bk2_block': 512}
+                'mixed_waves': 2, 'bk2_block': 512, 'coop_lps': 32}
 
 
 def generate(tables, info, opts=None):
@@ -1075,8 +1078,10 @@ def generate(tables, info, opts=None):
     if o.get('rolled') or int(tables['n_species']) > emit_rolled.ROLL_N:
         # large mechanism: loop forms (private per-lane species arrays, uniform table loads);
         # no MFMA-accumulation variant and no surrogate (its width is fixed at 64)
-        parts.append(emit_rolled.emit_bk1(tables, info, 'fp64'))
-        parts.append(emit_rolled.emit_bk1(tables, info, 'mixed_valu'))
+        # production rates: several lanes per state, species arrays in LDS (emit_coop)
+        from . import emit_coop
+        parts.append(emit_coop.emit_bk1(tables, info, 'fp64', o['coop_lps']))
+        parts.append(emit_coop.emit_bk1(tables, info, 'mixed_valu', o['coop_lps']))
         if 'cond' in tables:
             parts.append(emit_rolled.emit_bk2(tables, info, False))
             parts.append(emit_rolled.emit_bk2(tables, info, True))

@@ -2,11 +2,14 @@
 
 The straight-line kernels of emit_hip keep every species quantity in registers and expand
 every pair term; past ~100 species that code no longer fits the register file or the
-instruction cache (and takes hours to compile). The rolled forms keep the per-state
-species arrays in per-lane private memory (scratch, interleaved across lanes so every
-array access of a wavefront is one contiguous line), read the mechanism's coefficients as
-uniform (scalar-cache) loads from __device__ tables, and loop over species, pairs and
-reactions. Same arithmetic and summation order as the straight-line forms:
+instruction cache (and takes hours to compile). The rolled transport kernels keep the
+per-state species arrays in per-lane private memory (scratch, interleaved across lanes so
+every array access of a wavefront is one contiguous line), read the mechanism's
+coefficients as uniform (scalar-cache) loads from __device__ tables, and loop over species
+and pairs. The production-rate kernels of these mechanisms are emit_coop's (several lanes
+per state, arrays in LDS); this module keeps their reaction tables (bk1_data), the shared
+falloff code (emit_falloff) and the one-state loop core of the kinetix_species_rates device
+routine (emit_bk1_core). Same arithmetic and summation order as the straight-line forms:
 
   BK2: kinetix/core/mix_transport.py:261-471 and benchmark/okl/transportProps.okl:11-47
        (symmetric diffusion pair loop, Wilke viscosity, conductivity mixing rule);
@@ -173,8 +176,10 @@ def emit_bk2(t, info, pstate=False):
     return S.text()
 
 
-def _bk1_tables(t, pre):
-    """Reaction descriptors of the rolled BK1 (one row per reaction)."""
+def bk1_data(t):
+    """Reaction descriptors of the loop-form BK1 as arrays (one row per reaction): the
+    reactant/product molecule lists, net stoichiometry, metadata, FP64/FP32 rate
+    coefficients, efficiency sets and PLOG points."""
     N, R = int(t['n_species']), int(t['n_reactions'])
     nu_f, nu_r = np.asarray(t['nu_f']).reshape(R, N), np.asarray(t['nu_r']).reshape(R, N)
     kind, rev = np.asarray(t['kind']), np.asarray(t['reversible'])
@@ -250,6 +255,19 @@ def _bk1_tables(t, pre):
                 for A, b, Ta in rates:
                     plk += [math.log2(A), b, Ta * L2E]
         plo.append(len(plP))
+    return dict(flt=flt, plo=np.array(plo), plP=np.array(plP), pllnP=np.array(pllnP),
+                plrcp=np.array(plrcp), plko=np.array(plko), plnk=np.array(plnk), plk=np.array(plk),
+                fsp=f_sp, rsp=r_sp, nsp=n_sp, nnu=n_nu, meta=meta, dbl=dbl,
+                setoff=np.array(set_off), setsp=np.array(set_sp), sete=np.array(set_e),
+                nsets=len(sets), MX=MX, MS=MS)
+
+
+def _bk1_tables(t, pre):
+    """Declarations of the loop-form BK1 tables (bk1_data), prefixed pre."""
+    d = bk1_data(t)
+    flt, plo, plP, pllnP, plrcp = d['flt'], d['plo'], d['plP'], d['pllnP'], d['plrcp']
+    plko, plnk, plk, f_sp, r_sp, n_sp, n_nu = d['plko'], d['plnk'], d['plk'], d['fsp'], d['rsp'], d['nsp'], d['nnu']
+    meta, dbl, set_off, set_sp, set_e = d['meta'], d['dbl'], d['setoff'], d['setsp'], d['sete']
     out = [_table(f'{pre}_flt', flt, 'float'),
            _table(f'{pre}_plo', np.array(plo), 'int'), _table(f'{pre}_plP', np.array(plP)),
            _table(f'{pre}_pllnP', np.array(pllnP)), _table(f'{pre}_plrcp', np.array(plrcp)),
@@ -262,7 +280,7 @@ def _bk1_tables(t, pre):
            _table(f'{pre}_rcpW', t['rcpW']), _table(f'{pre}_W', t['W']),
            _table(f'{pre}_tmid', t['T_mid']), _table(f'{pre}_glo', t['g_lo']), _table(f'{pre}_ghi', t['g_hi']),
            _table(f'{pre}_hlo', t['h_lo']), _table(f'{pre}_hhi', t['h_hi'])]
-    return out, len(sets), MX, MS
+    return out, d['nsets'], d['MX'], d['MS']
 
 
 def plog_helpers(pre):
@@ -291,6 +309,56 @@ def plog_helpers(pre):
     H(f'return L1 + (L2 - L1) * ((lnP - {pre}_pllnP[i]) * {pre}_plrcp[i]);')
     H('}', 0)
     return H.text().split('\n')
+
+
+def emit_falloff(S, pre, mixed):
+    """Third-body / falloff factor of one reaction, shared by the loop forms: multiplies F by
+    M (three-body) or by Pr/(1+Pr) * F_cent-term (Lindemann, Troe, SRI), as the straight-line
+    forms do. In scope: kind, M (the reaction's third-body concentration), dp (a row whose
+    [3..12] hold the low-pressure Arrhenius offsets and Troe/SRI parameters), fp (FP32 copies,
+    [3..5]), T, rcpT, log2T, Tf, rcpTf, log2Tf and F; kind is neither elementary nor PLOG.
+    Helpers of prefix pre (pow2) precede it."""
+    S(f'    if (kind == {M.THREE_BODY}) {{')
+    S('      F *= M;')
+    S('    } else {')
+    if mixed:
+        S('      // clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32')
+        S('      // the FP32 exponent is capped at 100 so exp2f stays finite (no 0 * inf at M = 0)')
+        S('      const float Pr = fminf((float)M * __builtin_amdgcn_exp2f(fminf(fp[3] + fp[4] * log2Tf - fp[5] * rcpTf, 100.0f)), 1e30f);')
+        S('      const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
+        S('      float Fc = 1.0f;')
+        S(f'      if (kind == {M.TROE}) {{')
+        S('        const float A_ = (float)dp[6], rT3 = (float)dp[7], rT1 = (float)dp[8];')
+        S('        const float term3 = dp[9] >= 0.0 ? __builtin_amdgcn_exp2f((float)(-dp[9] * 1.4426950408889634) * rcpTf) : 0.0f;')
+        S('        const float logFc = 0.3010299956639812f * __builtin_amdgcn_logf((1.0f - A_) * '
+          '__builtin_amdgcn_exp2f(-rT3 * 1.4426950408889634f * Tf) + A_ * __builtin_amdgcn_exp2f(-rT1 * 1.4426950408889634f * Tf) + term3);')
+        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+        S('        const float tc = -0.4f - 0.67f * logFc + logPr;')
+        S('        const float tf = tc * __builtin_amdgcn_rcpf(0.75f - 1.27f * logFc - 0.14f * tc + 1e-37f);')
+        S('        Fc = __builtin_amdgcn_exp2f(3.3219280948873622f * logFc * __builtin_amdgcn_rcpf(1.0f + tf * tf));')
+        S(f'      }} else if (kind == {M.SRI}) {{')
+        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
+        S('        Fc = (float)dp[9] * powf(Tf, (float)dp[10]) * powf((float)dp[6] * expf((float)(-dp[7]) * rcpTf) + '
+          'expf((float)(-dp[8]) * Tf), __builtin_amdgcn_rcpf(1.0f + logPr * logPr));')
+        S('      }')
+        S('      F *= (double)(Pr1 * Fc);')
+    else:
+        S(f'      const double Pr = M * {pre}_pow2(dp[3] + dp[4] * log2T - dp[5] * rcpT);')
+        S('      const double Pr1 = Pr / (1.0 + Pr);')
+        S('      double Fc = 1.0;')
+        S(f'      if (kind == {M.TROE}) {{')
+        S('        const double term3 = dp[9] >= 0.0 ? exp(-dp[9] * rcpT) : 0.0;')
+        S('        const double logFc = log10((1.0 - dp[6]) * exp(-dp[7] * T) + dp[6] * exp(-dp[8] * T) + term3);')
+        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
+        S('        const double tc = -0.4 - 0.67 * logFc + logPr;')
+        S(f'        const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {E.CFLOAT_MIN});')
+        S('        Fc = exp10(logFc / (1.0 + tf * tf));')
+        S(f'      }} else if (kind == {M.SRI}) {{')
+        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
+        S('        Fc = dp[9] * pow(T, dp[10]) * pow(dp[6] * exp(-dp[7] * rcpT) + exp(-dp[8] * T), 1.0 / (1.0 + logPr * logPr));')
+        S('      }')
+        S('      F *= Pr1 * Fc;')
+    S('    }')
 
 
 def emit_bk1_core(S, t, pre, nsets, MX, MS, mixed):
@@ -340,47 +408,7 @@ def emit_bk1_core(S, t, pre, nsets, MX, MS, mixed):
     S(f'  double F = {pre}_pow2(ex);')
     S(f'  if (kind != 0 && kind != {M.PLOG}) {{')
     S('    const double M = mt[6] == 3 ? Ms[mt[7]] : (mt[6] == 2 ? Ccol[mt[7]] : Cm);')
-    S(f'    if (kind == {M.THREE_BODY}) {{')
-    S('      F *= M;')
-    S('    } else {')
-    if mixed:
-        S('      // clamped below FLT_MAX: Pr / (1 + Pr) must stay finite (-> 1) in FP32')
-        S('      // the FP32 exponent is capped at 100 so exp2f stays finite (no 0 * inf at M = 0)')
-        S('      const float Pr = fminf((float)M * __builtin_amdgcn_exp2f(fminf(fp[3] + fp[4] * log2Tf - fp[5] * rcpTf, 100.0f)), 1e30f);')
-        S('      const float Pr1 = Pr * __builtin_amdgcn_rcpf(1.0f + Pr);')
-        S('      float Fc = 1.0f;')
-        S(f'      if (kind == {M.TROE}) {{')
-        S('        const float A_ = (float)dp[6], rT3 = (float)dp[7], rT1 = (float)dp[8];')
-        S('        const float term3 = dp[9] >= 0.0 ? __builtin_amdgcn_exp2f((float)(-dp[9] * 1.4426950408889634) * rcpTf) : 0.0f;')
-        S('        const float logFc = 0.3010299956639812f * __builtin_amdgcn_logf((1.0f - A_) * '
-          '__builtin_amdgcn_exp2f(-rT3 * 1.4426950408889634f * Tf) + A_ * __builtin_amdgcn_exp2f(-rT1 * 1.4426950408889634f * Tf) + term3);')
-        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
-        S('        const float tc = -0.4f - 0.67f * logFc + logPr;')
-        S('        const float tf = tc * __builtin_amdgcn_rcpf(0.75f - 1.27f * logFc - 0.14f * tc + 1e-37f);')
-        S('        Fc = __builtin_amdgcn_exp2f(3.3219280948873622f * logFc * __builtin_amdgcn_rcpf(1.0f + tf * tf));')
-        S(f'      }} else if (kind == {M.SRI}) {{')
-        S('        const float logPr = 0.3010299956639812f * __builtin_amdgcn_logf(Pr + 1e-37f);')
-        S('        Fc = (float)dp[9] * powf(Tf, (float)dp[10]) * powf((float)dp[6] * expf((float)(-dp[7]) * rcpTf) + '
-          'expf((float)(-dp[8]) * Tf), __builtin_amdgcn_rcpf(1.0f + logPr * logPr));')
-        S('      }')
-        S('      F *= (double)(Pr1 * Fc);')
-    else:
-        S(f'      const double Pr = M * {pre}_pow2(dp[3] + dp[4] * log2T - dp[5] * rcpT);')
-        S('      const double Pr1 = Pr / (1.0 + Pr);')
-        S('      double Fc = 1.0;')
-        S(f'      if (kind == {M.TROE}) {{')
-        S('        const double term3 = dp[9] >= 0.0 ? exp(-dp[9] * rcpT) : 0.0;')
-        S('        const double logFc = log10((1.0 - dp[6]) * exp(-dp[7] * T) + dp[6] * exp(-dp[8] * T) + term3);')
-        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
-        S('        const double tc = -0.4 - 0.67 * logFc + logPr;')
-        S(f'        const double tf = tc / (0.75 - 1.27 * logFc - 0.14 * tc + {E.CFLOAT_MIN});')
-        S('        Fc = exp10(logFc / (1.0 + tf * tf));')
-        S(f'      }} else if (kind == {M.SRI}) {{')
-        S(f'        const double logPr = log10(Pr + {E.CFLOAT_MIN});')
-        S('        Fc = dp[9] * pow(T, dp[10]) * pow(dp[6] * exp(-dp[7] * rcpT) + exp(-dp[8] * T), 1.0 / (1.0 + logPr * logPr));')
-        S('      }')
-        S('      F *= Pr1 * Fc;')
-    S('    }')
+    emit_falloff(S, pre, mixed)
     S('  }')
     S('  double q;')
     S('  if (mt[1]) {')
@@ -393,64 +421,3 @@ def emit_bk1_core(S, t, pre, nsets, MX, MS, mixed):
     S('  }')
     S(f'  for (int i = 0; i < mt[4]; ++i) wd[{pre}_nsp[{MS} * r + i]] += {pre}_nnu[{MS} * r + i] * q;')
     S('}')
-
-
-def emit_bk1(t, info, variant='fp64'):
-    """Rolled BK1. variant 'fp64': FP64 exponentials and falloff; 'mixed_valu': Arrhenius
-    exponent in FP32, its sum with the FP64 Gibbs terms through kx_pow2 (FP64 rint, FP32
-    exp2 of the fraction), falloff in FP32; Gibbs terms, net rates and the wdot sums in
-    FP64 -- the straight-line forms' split."""
-    N, R = int(t['n_species']), int(t['n_reactions'])
-    name = f'kx_bk1_{variant}'
-    pre = f'{name}_r'
-    mixed = variant != 'fp64'
-    tabs, nsets, MX, MS = _bk1_tables(t, pre)
-    S = E._Src()
-    for line in tabs:
-        S(line, 0)
-    S(f'__device__ __forceinline__ double {pre}_pow2(double y)', 0)
-    S('{', 0)
-    if mixed:
-        S('const double n = __builtin_rint(y);')
-        S('const float m = __builtin_amdgcn_exp2f((float)(y - n));')
-        S('return __builtin_ldexp((double)m, (int)n);')
-    else:
-        S('return exp2(y);')
-    S('}', 0)
-    for line in plog_helpers(pre):
-        S(line, 0)
-    S(f'extern "C" __global__ void __launch_bounds__({E.BLOCK})', 0)
-    S(f'{name}(const int n_states, const long long offsetT, const long long offset,', 0)
-    S('            const double pressure_R, const double pressure_,', 0)
-    S('            const double* __restrict__ state, double* __restrict__ rates, const double Tref)', 0)
-    S('{', 0)
-    S(f'const long long id = (long long)blockIdx.x * {E.BLOCK} + threadIdx.x;')
-    S('if (id >= n_states) return;')
-    S('const double P = pressure_, lnP = log(P);   // PLOG reactions (uniform per launch)')
-    S('(void)P; (void)lnP;')
-    S('const double T = Tref * state[id];')
-    S('const double rcpT = 1.0 / T;')
-    S('const double lnT = log(T);')
-    S('const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;')
-    S('const double log2T = lnT * 1.4426950408889634;')
-    S('const float Tf = (float)T, rcpTf = (float)rcpT, log2Tf = (float)log2T;')
-    S('(void)Tf; (void)rcpTf; (void)log2Tf;')
-    S(f'double w[{N}], wd[{N}];   // per-lane private arrays')
-    S('double rcpMbar = 0.0;')
-    S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) {{ const double c = fmax(0.0, state[id + offsetT + k * offset]) * {pre}_rcpW[k]; w[k] = c; rcpMbar += c; }}')
-    S('const double rho = pressure_R * rcpT / rcpMbar;')
-    S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) w[k] *= rho;')
-    emit_bk1_core(S, t, pre, nsets, MX, MS, mixed)
-    S('// ---- mass production rates and heat release rate')
-    S('double sum_h = 0.0;')
-    S('#pragma unroll 1')
-    S(f'for (int k = 0; k < {N}; ++k) {{')
-    S(f'  rates[id + offsetT + k * offset] = {pre}_W[k] * wd[k];')
-    S(f'  const double* a = (T > {pre}_tmid[k] ? {pre}_hhi : {pre}_hlo) + 6 * k;')
-    S('  sum_h += wd[k] * (a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4 + a[5] * rcpT);')
-    S('}')
-    S(f'rates[id] = {repr(-float(t["R"]))} * T * sum_h;')
-    S('}', 0)
-    return S.text()

@@ -235,6 +235,48 @@ def test_wilke_mfma_fragments_reproduce_the_viscosity(gri, oracle):
     assert np.max(np.abs(mu / v0 - 1)) < 1e-13
 
 
+@pytest.mark.parametrize('mech', ['synth160', 'synth40cov_rolled', 'gri30_rolled'])
+def test_cooperative_descriptors_round_trip(mech):
+    """The packed 64-byte reaction descriptors of the cooperative loop-form kernels
+    (emit_coop.packed) decode, field by field, to the loop-form tables they are built from:
+    reactant/product molecules (padding = species N, the slot with w = 1 and g = 0), net
+    species and signed coefficients, kind / reversibility / counts / third-body kind and
+    index, falloff rows, and every reaction exactly once."""
+    from kinetix_amd.gen import emit_coop, tables
+    _, t = tables.load(os.path.join(ROOT, 'kinetix_amd', 'mechs', mech))
+    N, R = int(t['n_species']), int(t['n_reactions'])
+    P = emit_coop.packed(t)
+    d = P['data']
+    assert sorted(P['order']) == list(range(R))
+    byte = lambda w, i: (int(w) >> (8 * i)) & 255
+    nfo = 0
+    for i, r in enumerate(P['order']):
+        w = P['rxi'][i]
+        kind, rev, nf, nr, nnet, dnu, mk, midx = (int(x) for x in d['meta'][r])
+        m = int(w[2])
+        assert (m & 15, (m >> 4) & 1, (m >> 5) & 7, (m >> 8) & 7, (m >> 11) & 15,
+                ((m >> 15) & 15) - 8, (m >> 19) & 3) == (kind, rev, nf, nr, nnet, dnu, mk)
+        fl = [byte(w[0], j) for j in range(4)]
+        rl = [byte(w[1], j) for j in range(4)]
+        assert fl[:nf] == [int(k) for k in d['fsp'][r][:nf]] and all(k == N for k in fl[nf:P['mx']])
+        assert rl[:nr] == [int(k) for k in d['rsp'][r][:nr]] and all(k == N for k in rl[nr:P['mx']])
+        sp = [byte(w[4 + j // 4], j % 4) for j in range(nnet)]
+        nu = [byte(w[6 + j // 4], j % 4) - 256 * (byte(w[6 + j // 4], j % 4) > 127) for j in range(nnet)]
+        assert sp == [int(k) for k in d['nsp'][r][:nnet]]
+        assert nu == [int(round(float(v))) for v in d['nnu'][r][:nnet]]
+        assert int(w[3]) == (r if kind == M.PLOG else midx)
+        np.testing.assert_array_equal(P['rxd'][i], d['dbl'][r, :3])
+        if kind in (M.LINDEMANN, M.TROE, M.SRI):
+            fs = m >> 21
+            assert fs == nfo
+            np.testing.assert_array_equal(P['rxfo'][10 * fs + 3:10 * fs + 13], d['dbl'][r, 3:13])
+            np.testing.assert_array_equal(P['rxff'][3 * fs + 3:3 * fs + 6], d['flt'][r, 3:6])
+            nfo += 1
+    # sorted by kind: the lanes of one iteration mostly share a branch
+    kinds = [int(d['meta'][r, 0]) for r in P['order']]
+    assert kinds == sorted(kinds)
+
+
 def test_liveness_order(gri):
     """The BK1 reaction order is a permutation, deterministic, and lowers the modelled register
     peak below both the generator's grouped order and the all-live layout (w, g and wd of
