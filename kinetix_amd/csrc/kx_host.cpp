@@ -46,6 +46,8 @@ struct Mech {
   hipModule_t module = nullptr;
   std::map<std::string, hipFunction_t> fn;
   std::map<std::string, unsigned> block;     // each kernel's launch bound (threads per block)
+  int queue_lanes = 1;                        // lanes per queued state of kx_bk2_fp64_queue
+  long long queue_grid_cap = 0;               // its workgroups resident at once (strided form)
   int n_species = 0, n_active = 0, n_reactions = 0;
   std::vector<std::string> names;
   std::vector<double> W;
@@ -119,17 +121,34 @@ int read_meta(Mech* m, const std::string& path)
 
 // Launch helper: packs the kernel arguments; 1-D grid, one state per thread, in workgroups
 // of the kernel's own launch bound (the generator picks it per kernel).
-int launch(Mech* m, const char* name, int n_states, void* stream, void** args)
+// One lane per element of n_lanes (a state, or a queue slot times the lanes per state).
+int launch(Mech* m, const char* name, long long n_lanes, void* stream, void** args)
 {
   auto it = m->fn.find(name);
   if (it == m->fn.end() || !it->second)
     return fail(KX_ERR_NO_KERNEL, std::string("kernel ") + name + " not in " + m->dir);
-  if (n_states == 0) return KX_OK;
+  if (n_lanes == 0) return KX_OK;
   KX_HIP(hipSetDevice(m->device));
   const unsigned block = m->block.at(name);
-  const unsigned grid = (unsigned)((n_states + (long long)block - 1) / block);
+  const long long g = (n_lanes + (long long)block - 1) / block;
+  if (g > 0x7fffffffLL) return fail(KX_ERR_INVALID_ARG, "grid too large");
+  const unsigned grid = (unsigned)g;
   KX_HIP(hipModuleLaunchKernel(it->second, grid, 1, 1, block, 1, 1, 0, (hipStream_t)stream,
                                args, nullptr));
+  return KX_OK;
+}
+
+// An explicit grid of workgroups of the kernel's launch bound.
+int launch_grid(Mech* m, const char* name, long long grid, void* stream, void** args)
+{
+  auto it = m->fn.find(name);
+  if (it == m->fn.end() || !it->second)
+    return fail(KX_ERR_NO_KERNEL, std::string("kernel ") + name + " not in " + m->dir);
+  if (grid == 0) return KX_OK;
+  if (grid < 0 || grid > 0x7fffffffLL) return fail(KX_ERR_INVALID_ARG, "grid out of range");
+  KX_HIP(hipSetDevice(m->device));
+  KX_HIP(hipModuleLaunchKernel(it->second, (unsigned)grid, 1, 1, m->block.at(name), 1, 1, 0,
+                               (hipStream_t)stream, args, nullptr));
   return KX_OK;
 }
 
@@ -174,6 +193,36 @@ static int finish_init(Mech* m, kx_mech_t* out)
     }
     m->fn[k] = f;
     m->block[k] = (unsigned)bound;
+  }
+  // the queue kernel may share each state among several lanes; it then exports how many
+  {
+    hipDeviceptr_t p = nullptr;
+    size_t bytes = 0;
+    if (hipModuleGetGlobal(&p, &bytes, m->module, "kx_bk2q_lanes") == hipSuccess &&
+        bytes == sizeof(int)) {
+      int lanes = 0;
+      if (hipMemcpyDtoH(&lanes, p, sizeof(int)) != hipSuccess || lanes < 1 ||
+          !m->block.count("kx_bk2_fp64_queue") || m->block["kx_bk2_fp64_queue"] % lanes) {
+        (void)hipGetLastError();
+        (void)hipModuleUnload(m->module);
+        delete m;
+        return fail(KX_ERR_HIP, "unusable kx_bk2q_lanes");
+      }
+      m->queue_lanes = lanes;
+      // the strided queue kernel runs on a grid capped at the workgroups resident at once
+      int ncu = 0, per_cu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) !=
+              hipSuccess || ncu <= 0)
+        ncu = 1;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, m->fn["kx_bk2_fp64_queue"], (int)m->block["kx_bk2_fp64_queue"], 0) !=
+              hipSuccess || per_cu <= 0)
+        per_cu = 1;
+      (void)hipGetLastError();
+      m->queue_grid_cap = (long long)ncu * per_cu;
+    } else {
+      (void)hipGetLastError();
+    }
   }
   std::lock_guard<std::mutex> lk(g_mu);
   *out = g_next++;
@@ -549,7 +598,16 @@ int kx_sur_transport(kx_mech_t h, int n, int offsetT, int offset, const double* 
     void* args[] = {&n_audit, &queue, &counters, &flags, &audit_tol, &audit_tol_rhoD, &oT, &o,
                     (void*)&d_p,
                     (void*)&d_state, &d_cond, &d_visc, &d_rhoD, &d_Dkm, &Tref};
-    if ((rc = launch(m, "kx_bk2_fp64_queue", n_audit + n, stream, args))) return rc;
+    const long long longest = (long long)n_audit + n;
+    if (m->queue_lanes > 1) {
+      // the cooperative form strides over the queue: a grid of the workgroups resident at
+      // once, or fewer when even the longest possible queue needs fewer
+      const long long per_wg = m->block.at("kx_bk2_fp64_queue") / m->queue_lanes;
+      const long long grid = std::min((longest + per_wg - 1) / per_wg, m->queue_grid_cap);
+      if ((rc = launch_grid(m, "kx_bk2_fp64_queue", grid, stream, args))) return rc;
+    } else if ((rc = launch(m, "kx_bk2_fp64_queue", longest, stream, args))) {
+      return rc;
+    }
   }
   // 4. whole-call fallback, decided on the device: counters[3] = n when the failed audits
   //    exceed max_audit_fail * n_audit, else 0; the plain kernel for all states runs with

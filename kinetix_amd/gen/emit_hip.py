@@ -884,7 +884,9 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
     threads (8 waves) is what gives 2 waves/SIMD: 8.4 ms instead of 14.5 at 256 threads for
     4M states, despite ~160 VGPRs spilled to scratch; 768 and 1024 spill far more and are
     2-3x slower. The queue form is latency-bound (a few 10^4 states): 256 threads, one wave per
-    SIMD on every CU, each wave free to use 372 of the 512 registers without a spill."""
+    SIMD on every CU, each wave free to use 372 of the 512 registers without a spill. It is
+    emitted only for a mechanism whose tables do not fit the cooperative queue kernel's LDS
+    (emit_bk2_queue, 16 lanes per state)."""
     N = int(t['n_species'])
     S = _Src()
     S(f'extern "C" __global__ void __launch_bounds__({block})', 0)
@@ -1101,8 +1103,13 @@ def generate(tables, info, opts=None):
                                 lds=o['lds_pool'], block=o['bk2_block']))
         parts.append(_with_pool('kx_bk2_fp64_pstate_k', _emit_bk2_fp64, tables, info, True, o['bk2_block'],
                                 lds=o['lds_pool'], block=o['bk2_block']))
-        parts.append(_with_pool('kx_bk2_fp64_queue_k', _emit_bk2_fp64, tables, info, True,
-                                256, True, lds=o['lds_pool'], block=256))
+        from . import emit_bk2_queue as BQ
+        if BQ.lds_bytes(tables) <= 160 * 1024:
+            # several lanes per state: the pass over a few 10^4 queued states is latency-bound
+            parts.append(BQ.emit(tables, info))
+        else:
+            parts.append(_with_pool('kx_bk2_fp64_queue_k', _emit_bk2_fp64, tables, info, True,
+                                    256, True, lds=o['lds_pool'], block=256))
         from . import emit_bk2_mfma as BM
         # the constant pool goes to LDS only beside the Wilke tables within 160 KB per CU
         parts.append(_with_pool('kx_bk2_fp64_mfma_k', BM.emit, tables, info, lds=o['lds_pool'],

@@ -8,7 +8,10 @@ kernels in kinetix_amd/csrc/kx_mlp.hip) learns the standardised residual:
   * mu baseline: the Wilke mixing rule (the formula the reference uses,
     kinetix/core/mix_transport.py:297-357) with its three constant pair matrices C1^2,
     C1 C2, C2^2 replaced by rank-r factorisations (their singular values fall by ~30x per
-    rank; r = 8 reproduces them to ~1e-9 relative), so the O(N^2) pair sum costs O(r N);
+    rank), so the O(N^2) pair sum costs O(r N). For GRI-3.0, r = 5 (sixth singular values
+    <= 2.5e-6 of the first) keeps mu within 2.4e-6 of the full-rank rule over 20,000
+    transport states, against the 1e-3 the surrogate answers to; r = 8 would be 1e-11 at
+    ~950 more FP32 FMAs per wavefront of 64 states;
   * lambda baseline: the reference's O(N) conductivity mixing rule (mix_transport.py:261-294);
   * D_km baseline: the binary diffusion coefficient of species k in N2 (N2 in O2), at the
     state's T and p, from the same pair fits.
@@ -33,7 +36,7 @@ import numpy as np
 from . import emit_hip as E
 
 IN = 64            # padded MLP input / output width
-RANK = 8
+RANK = 5
 BLOCK = 256
 
 
@@ -149,12 +152,14 @@ def emit(t, info):
 
     def hull_and_inputs(store):
         """out = 1 if any raw input lies outside the hull box (or is NaN); store(i, expr)
-        receives the standardised input i."""
+        receives the standardised input i: (raw - mean) times the hardware reciprocal of the
+        standard deviation (~1 ulp FP32, far below the BF16 rounding the MLP input takes;
+        a correctly rounded divide costs ~10 instructions per input)."""
         S('int out = 0;')
         S('#pragma unroll')
         S(f'for (int i = 0; i < {N + 2}; ++i) {{')
         S(f'  out |= (raw[i] < norm[{2 * IN} + i]) | (raw[i] > norm[{3 * IN} + i]) | (raw[i] != raw[i]);')
-        S('  ' + store('i', f'(raw[i] - norm[i]) / norm[{IN} + i]'))
+        S('  ' + store('i', f'(raw[i] - norm[i]) * __builtin_amdgcn_rcpf(norm[{IN} + i])'))
         S('}')
 
     def rcp_mbar():

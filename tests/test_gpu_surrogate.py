@@ -20,6 +20,21 @@ dev = torch.device('cuda')
 SMOKE_TOL_RHOD = 0.2
 
 
+# States the surrogate queues (audit samples, out-of-hull states) are recomputed by the plain
+# FP64 pass with several lanes per state: its group sums over species (mu, lambda) add in
+# another order than the one-lane-per-state plain kernel, so the two agree to rounding, not
+# bit for bit.
+QUEUE_RTOL = 1e-13
+
+
+def same_as_plain(a, b):
+    """a (queued states' outputs) equals b (the plain kernel's) to QUEUE_RTOL."""
+    assert bool(torch.isfinite(a).all()) and bool(torch.isfinite(b).all())
+    if a.numel() == 0:
+        return True
+    return float(((a - b).abs() / b.abs().clamp_min(1e-300)).max()) <= QUEUE_RTOL
+
+
 def t32(a):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
 
@@ -161,7 +176,7 @@ def test_out_of_hull_states_take_the_plain_result(trained, gri):
     assert res['out_of_hull'] >= 10
     for a, b in zip(got, ref):
         a_, b_ = a.view(-1, 1000)[:, :10], b.view(-1, 1000)[:, :10]
-        assert torch.equal(a_, b_)
+        assert same_as_plain(a_, b_)
 
 
 def test_audit_failure_falls_back_whole_call(trained, gri):
@@ -246,13 +261,13 @@ def test_out_of_hull_beyond_one_compact_pass(trained, gri):
     flagged[:4700] = True
     for a, b in zip(got, ref):
         a_, b_ = a.view(-1, n)[:, flagged], b.view(-1, n)[:, flagged]
-        assert torch.equal(a_, b_)
+        assert same_as_plain(a_, b_)
     # audited states that were in hull also carry the plain result
     aud = np.zeros(n, dtype=bool)
     aud[::100] = True
     aud &= ~flagged
     for a, b in zip(got, ref):
-        assert torch.equal(a.view(-1, n)[:, aud], b.view(-1, n)[:, aud])
+        assert same_as_plain(a.view(-1, n)[:, aud], b.view(-1, n)[:, aud])
 
 
 def test_enqueue_only_call_matches(trained, gri):
@@ -321,7 +336,7 @@ def test_enqueue_only_call_replays_from_a_graph(trained, gri):
 def test_audit_sampling_edges(trained, gri, n, stride, offset):
     """The plain pass over the audit queue at its edges: every state audited (stride 1), an odd
     count with an offset, an offset past the last state, a single state. Audited states carry
-    the plain kernel's result exactly; the count matches the sampling rule."""
+    the plain kernel's result (to rounding); the count matches the sampling rule."""
     info, t = gri
     kx, sur, _ = trained
     s, p = states.sample_transport_set(n, info['species'], t['W'], seed=40 + n)
@@ -331,5 +346,5 @@ def test_audit_sampling_edges(trained, gri, n, stride, offset):
     assert res['audited'] == int(aud.sum())
     for a, b in zip(got, ref):
         a_, b_ = a.view(-1, n)[:, aud], b.view(-1, n)[:, aud]
-        assert torch.equal(a_, b_)
+        assert same_as_plain(a_, b_)
         assert torch.all(torch.isfinite(a.view(-1, n)))
