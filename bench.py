@@ -33,7 +33,9 @@ PEAK_BF16 = 2500.0    # TFLOP/s, dense BF16 MFMA
 # same figure for scale.
 PEAK_TRANSC = 256 * 4 * 16 / 4 * 2.4e9 / 1e12   # T/s
 CPU_SHARE = 16        # host threads one GPU's lease provides
-MLP_FLOPS = 2 * (55 * 256 + 256 * 256 * 2 + 256 * 55)   # useful MACs x 2 per state
+def mlp_flops(H):
+    """Useful MACs x 2 per state of the surrogate MLP 55 -> H -> H -> H -> 55."""
+    return 2 * (55 * H + 2 * H * H + H * 55)
 
 
 def SUR_BYTES(N):
@@ -369,12 +371,15 @@ def main():
         (profiles/traffic.json) for this kernel at this state count, when one exists."""
         tr = traffic.get(f'{name}@{n}')
         if name == 'kx_sur_infer':
-            fl = MLP_FLOPS * n / (ms * 1e-3) / 1e12
+            from kinetix_amd.surrogate import hidden_width
+            H = hidden_width()
+            fl = mlp_flops(H) * n / (ms * 1e-3) / 1e12
             gbs = SUR_BYTES(N) * n / (ms * 1e-3) / 1e9
             return {'bound': 'mfma', 'pipe': 'BF16 MFMA (dense peak 2.5 PFLOP/s)',
+                    'binding': f'VALU issue: the FP32 physics baselines and GELU beside an MLP of width {H}',
                     'achieved': round(fl, 3), 'peak': PEAK_BF16, 'unit': 'TFLOP/s',
                     'frac': round(fl / PEAK_BF16, 4), 'traffic': tr,
-                    'flops_per_state': MLP_FLOPS, 'bytes_per_state': SUR_BYTES(N),
+                    'flops_per_state': mlp_flops(H), 'bytes_per_state': SUR_BYTES(N),
                     'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
                     'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
         c = counts[name if name in counts else ('kx_bk2_fp64' if 'bk2' in name else 'kx_bk1_fp64')]

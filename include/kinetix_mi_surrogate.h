@@ -4,7 +4,7 @@
  *
  * Model: y = b(T, p, Y) + out_mean + out_std * MLP((x - in_mean) / in_std), with
  * y = (ln mu, ln lambda, ln D_1m .. ln D_Nm), x = (normalised T, normalised ln p, Y_1..Y_N)
- * padded to 64, b an FP32 physics baseline, MLP 64 -> 3x256 GELU -> 64. All buffers are HIP
+ * padded to 64, b an FP32 physics baseline, MLP 64 -> 3x128 GELU -> 64. All buffers are HIP
  * device pointers; all calls are asynchronous on `stream` (NULL = default stream).
  */
 #ifndef KINETIX_MI_SURROGATE_H
@@ -118,15 +118,16 @@ int kx_sur_transport_noalloc(kx_mech_t m, int n_states, int offsetT, int offset,
 
 /* ---- mechanism-independent MLP kernels ---- */
 
+int kx_mlp_width(void);             /* hidden width H (128) */
 int kx_mlp_n_params(void);          /* FP32 parameter block length */
 int kx_mlp_n_weights_bf16(void);    /* BF16 weight block length */
 
-/* Training forward pass, batch B (multiple of 64): acts receives Z1 H1 Z2 H2 Z3 H3 (B x 256
+/* Training forward pass, batch B (multiple of 64): acts receives Z1 H1 Z2 H2 Z3 H3 (B x H
  * each) and Y (B x 64). */
 int kx_mlp_forward(const float *params, int B, const float *X, float *acts, void *stream);
 
 /* Gradient of mean((Y - T)^2) over the first 55 outputs into grads (params layout).
- * work: 2 x B x 256 floats; loss_part: ceil(B * 64 / 256) floats of partial squared errors. */
+ * work: 2 x B x H floats; loss_part: ceil(B * 64 / 256) floats of partial squared errors. */
 int kx_mlp_backward(const float *params, int B, const float *X, const float *acts,
                     const float *T, float *grads, float *work, float *loss_part, void *stream);
 

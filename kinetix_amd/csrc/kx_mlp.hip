@@ -1,4 +1,4 @@
-// Hand-written kernels of the transport-surrogate MLP: 64 -> 256 -> 256 -> 256 -> 64
+// Hand-written kernels of the transport-surrogate MLP: 64 -> kH -> kH -> kH -> 64 (kH = 128)
 // (55 inputs / 55 outputs zero-padded to 64), GELU (tanh form) on the hidden layers.
 //
 // Training (FP32 master weights): tiled FP32 GEMM with either operand transposed, bias +
@@ -6,8 +6,8 @@
 // Inference: BF16 weights packed into MFMA fragment order (kx_mlp_wave.h) and the
 // wave-level forward pass of kx_mlp_wave.h, 64 states per wavefront.
 //
-// Parameter block (FP32, contiguous): W1[256][64] b1[256] W2[256][256] b2[256]
-// W3[256][256] b3[256] W4[64][256] b4[64].
+// Parameter block (FP32, contiguous): W1[kH][64] b1[kH] W2[kH][kH] b2[kH]
+// W3[kH][kH] b3[kH] W4[64][kH] b4[64].
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -268,6 +268,7 @@ int hip_status() { return hipGetLastError() == hipSuccess ? KX_OK : KX_ERR_HIP; 
 
 extern "C" {
 
+int kx_mlp_width(void) { return kH; }
 int kx_mlp_n_params(void) { return kParams; }
 int kx_mlp_n_weights_bf16(void) { return kWeightsBf16; }
 
@@ -285,7 +286,7 @@ int kx_selftest_mfma_bf16_k32_layout(float* d_out, void* stream)
   return hip_status();
 }
 
-// acts (FP32, 7 blocks of B x 256 = Z1 H1 Z2 H2 Z3 H3 and Y (B x 64) at the end)
+// acts (FP32, 6 blocks of B x kH = Z1 H1 Z2 H2 Z3 H3 and Y (B x 64) at the end)
 int kx_mlp_forward(const float* p, int B, const float* X, float* acts, void* stream)
 {
   if (!p || !X || !acts || B <= 0 || B % 64) return KX_ERR_INVALID_ARG;
@@ -305,7 +306,7 @@ int kx_mlp_forward(const float* p, int B, const float* X, float* acts, void* str
 }
 
 // Gradient of the masked MSE (first 55 outputs) w.r.t. all parameters, written to g
-// (same layout as p). work: 2 x B x 256 floats. loss_part: ceil(B*64/256) floats.
+// (same layout as p). work: 2 x B x kH floats. loss_part: ceil(B*64/256) floats.
 int kx_mlp_backward(const float* p, int B, const float* X, const float* acts, const float* T,
                     float* g, float* work, float* loss_part, void* stream)
 {
@@ -314,7 +315,7 @@ int kx_mlp_backward(const float* p, int B, const float* X, const float* acts, co
   hipStream_t s = (hipStream_t)stream;
   const size_t L = (size_t)B * kH;
   const float *Z1 = acts, *H1 = Z1 + L, *Z2 = H1 + L, *H2 = Z2 + L, *Z3 = H2 + L, *H3 = Z3 + L, *Yo = H3 + L;
-  float *dA = work, *dB = work + L;       // dH / dZ scratch (B x 256 each)
+  float *dA = work, *dB = work + L;       // dH / dZ scratch (B x kH each)
   float* dY = dB;                          // B x 64 fits in dB
   hipLaunchKernelGGL(mse_grad, dim3(grid1((size_t)B * kOut)), dim3(256), 0, s, B, kOut, kValidOut, Yo, T, dY, loss_part);
   // layer 4: dW4 = dY^T H3, db4, dH3 = dY W4

@@ -1,5 +1,5 @@
 // Wave-level BF16-MFMA forward pass of the transport-surrogate MLP
-// (64 -> 256 -> 256 -> 256 -> 64, GELU on the hidden layers) for 64 states per wavefront.
+// (64 -> kH -> kH -> kH -> 64, GELU on the hidden layers) for 64 states per wavefront.
 //
 // Layout choices (gfx950, v_mfma_f32_16x16x32_bf16, wave64):
 //  * every layer is computed transposed, Z^T = W H^T: the MFMA's A operand is a 16 x 32
@@ -25,9 +25,13 @@
 
 namespace kxmlp {
 
-constexpr int kIn = 64, kH = 256, kOut = 64, kValidOut = 55;
-// FP32 parameter block: W1[256][64] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
-// W4[64][256] b4[64]
+// Hidden width 128: the inference kernel's time is ~70 % MLP (its activation VALU work and
+// MFMA chains), so the width sets it -- 2.65 ms at 256, 2.13 at 192, 1.62 at 128 for 4M
+// states -- while the outputs' error is set by the BF16 rounding of the activations, not by
+// the width. A multiple of 64 (the training GEMM tiles).
+constexpr int kIn = 64, kH = 128, kOut = 64, kValidOut = 55;
+static_assert(kH % 64 == 0, "training GEMM tiles are 64 wide");
+// FP32 parameter block: W1[kH][64] b1[kH] W2[kH][kH] b2[kH] W3[kH][kH] b3[kH] W4[64][kH] b4[64]
 constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH;
 constexpr int kW3 = kB2 + kH, kB3 = kW3 + kH * kH, kW4 = kB3 + kH, kB4 = kW4 + kOut * kH;
 constexpr int kParams = kB4 + kOut;
@@ -75,8 +79,9 @@ __device__ __forceinline__ float gelu_fast(float x)
 }
 
 // activations of one pass (SB blocks of 16 states) for one layer: [k block][state block]
-// fragments. A pass of SB = 4 covers the wave's 64 states with 256 activation registers
-// (input + output layer); SB = 2 covers 32 states with 128, at twice the weight reads per state.
+// fragments, kH / 32 x SB x 4 registers per layer. A pass of SB = 4 covers the wave's 64
+// states (input + output layer: 2 kH / 8 x 4 registers, 128 at kH = 128); SB = 2 covers 32
+// states with half that, at twice the weight reads per state.
 template <int KB, int SB = 4>
 using Act = u32x4[KB][SB];
 
@@ -115,7 +120,7 @@ __device__ __forceinline__ void mfma_block(const u32x4 (&a)[KB], const f32x4& b,
 }
 
 template <int SB>
-__device__ __forceinline__ void gelu_block(const f32x4 (&acc)[SB], Act<8, SB>& out, int ob)
+__device__ __forceinline__ void gelu_block(const f32x4 (&acc)[SB], Act<kH / 32, SB>& out, int ob)
 {
 #pragma unroll
   for (int sb = 0; sb < SB; ++sb) {
@@ -137,7 +142,7 @@ __device__ __forceinline__ void load_block(const u32x4* __restrict__ wf, const f
 
 template <int KB, int SB>
 __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
-                                             const Act<KB, SB>& in, Act<8, SB>& out, int lane)
+                                             const Act<KB, SB>& in, Act<kH / 32, SB>& out, int lane)
 {
   constexpr int OB = kH / 16;
   u32x4 a0[KB], a1[KB];
@@ -162,7 +167,7 @@ __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const
 // y[ob][sb][e]: output feature 16 ob + 4 (lane / 16) + e of state 16 sb + lane % 16
 template <int SB>
 __device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
-                                             const Act<8, SB>& in, f32x4 (&y)[kOut / 16][SB], int lane)
+                                             const Act<kH / 32, SB>& in, f32x4 (&y)[kOut / 16][SB], int lane)
 {
   const int g = lane >> 4;
   u32x4 a_cur[kH / 32], a_nxt[kH / 32];
@@ -203,10 +208,10 @@ __device__ __forceinline__ void forward(const uint16_t* __restrict__ wfrag, cons
   const u32x4* wf = (const u32x4*)wfrag;
   Act<kIn / 32, SB> x0;
   load_input<XS, SB>(xs, x0, lane);
-  Act<8, SB> h1, h2;
+  Act<kH / 32, SB> h1, h2;
   hidden_layer<kIn / 32>(wf + kF1 / 8, p + kB1, x0, h1, lane);
-  hidden_layer<8>(wf + kF2 / 8, p + kB2, h1, h2, lane);
-  hidden_layer<8>(wf + kF3 / 8, p + kB3, h2, h1, lane);
+  hidden_layer<kH / 32>(wf + kF2 / 8, p + kB2, h1, h2, lane);
+  hidden_layer<kH / 32>(wf + kF3 / 8, p + kB3, h2, h1, lane);
   output_layer(wf + kF4 / 8, p + kB4, h1, y, lane);
 }
 

@@ -2,7 +2,7 @@
 
 The surrogate predicts the mixture-averaged transport outputs y = (ln mu, ln lambda,
 ln D_1m .. ln D_Nm) as  y = b(T, p, Y) + mean + std * MLP(x),  where b is a cheap FP32
-physics baseline and the MLP (55 -> 3x256 GELU -> 55, BF16 MFMA inference; hand-written
+physics baseline and the MLP (55 -> 3x128 GELU -> 55, BF16 MFMA inference; hand-written
 kernels in kinetix_amd/csrc/kx_mlp.hip) learns the standardised residual:
 
   * mu baseline: the Wilke mixing rule (the formula the reference uses,
@@ -80,10 +80,13 @@ def baseline_numpy(t, species, T, p, Y):
 XS = 68            # LDS row stride (bf16) of the staged MLP inputs in the fused kernel
 
 
-def emit(t, info):
+def emit(t, info, passes=1):
     N = int(t['n_species'])
     if N + 2 > IN:
         return ''                                   # surrogate width is fixed at 64
+    if passes not in (1, 2, 4):
+        raise ValueError('passes of the wavefront\'s 64 states: 1, 2 or 4')
+    sb_n = 4 // passes                              # 16-state blocks per pass
     sp = info['species']
     W, rcpW = t['W'], t['rcpW']
     lit = E.lit
@@ -241,9 +244,11 @@ def emit(t, info):
     dtab = np.array([t['diff'][k, pr[k]] for k in range(N)], dtype=np.float32)      # (N, 5)
     S(f'__device__ const float kx_sur_dpoly[{5 * N}] = {{' +
       ', '.join(f'{float(v)!r}f' for v in dtab.reshape(-1)) + '};', 0)
-    S('// One state per lane for the physics part; the MLP (kx_mlp_wave.h) in two passes of 32', 0)
-    S('// states per wavefront, so activations take 128 VGPRs and two waves share a SIMD (one', 0)
-    S('// wave\'s MFMAs overlap the other\'s feature and GELU VALU work). LDS carries the inputs', 0)
+    S(f'// One state per lane for the physics part; the MLP (kx_mlp_wave.h) in {passes} pass(es) of', 0)
+    S(f'// {64 // passes} states per wavefront (at hidden width 128 one pass of 64: 1.53 ms for 4M states', 0)
+    S('// against 1.64 in two passes of 32, which halve the activation registers but read every', 0)
+    S('// weight fragment twice); two waves share a SIMD (one wave\'s MFMAs overlap the other\'s', 0)
+    S('// feature and GELU VALU work). LDS carries the inputs', 0)
     S('// and four floats per state (ln mu and ln lambda baselines + output means, ln T,', 0)
     S('// 1.5 ln T - ln p) between the two layouts; the diffusion baselines are recomputed per', 0)
     S('// output from the FP32 fit table. States flagged out of hull (or beyond n_states) get', 0)
@@ -290,12 +295,12 @@ def emit(t, info):
     S('__syncthreads();')
     S('const int j = lane & 15, g = lane >> 4;')
     S('#pragma unroll 1')
-    S('for (int h = 0; h < 2; ++h) {')
-    S('kxmlp::f32x4 y[kxmlp::kOut / 16][2];')
-    S(f'kxmlp::forward<{XS}, 2>(wfrag, params, xs[wv] + 32 * h * {XS}, y, lane);')
+    S(f'for (int h = 0; h < {passes}; ++h) {{')
+    S(f'kxmlp::f32x4 y[kxmlp::kOut / 16][{sb_n}];')
+    S(f'kxmlp::forward<{XS}, {sb_n}>(wfrag, params, xs[wv] + {16 * sb_n} * h * {XS}, y, lane);')
     S('#pragma unroll')
-    S('for (int sb = 0; sb < 2; ++sb) {')
-    S('  const int s = 32 * h + 16 * sb + j;')
+    S(f'for (int sb = 0; sb < {sb_n}; ++sb) {{')
+    S(f'  const int s = {16 * sb_n} * h + 16 * sb + j;')
     S('  const long long sid = (long long)blockIdx.x * 256 + wv * 64 + s;')
     S('  if (fs[wv][s]) continue;')
     S('  const double rho = rs[wv][s];')

@@ -53,6 +53,7 @@ def _bind():
     L.kx_sur_transport.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, C.c_double,
                                    C.c_double, C.c_double, vp, vp, vp, vp, vp, C.c_size_t,
                                    C.POINTER(SurStats), vp]
+    L.kx_mlp_width.restype = ci
     L.kx_mlp_forward.argtypes = [vp, ci, vp, vp, vp]
     L.kx_mlp_backward.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kx_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
@@ -70,14 +71,19 @@ def _s():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def hidden_width():
+    """The library's hidden width H (the MLP is 64 -> H -> H -> H -> 64)."""
+    return int(lib().kx_mlp_width())
+
+
 def init_params(seed=0):
     """Glorot-uniform weights, zero biases; padded input columns / output rows are zero."""
     rng = np.random.default_rng(seed)
     n = lib().kx_mlp_n_params()
+    H = hidden_width()
     p = np.zeros(n, dtype=np.float32)
     o = 0
-    for (rows, cols, vr, vc) in ((256, 64, 256, VALID), (256, 256, 256, 256), (256, 256, 256, 256),
-                                 (64, 256, VALID, 256)):
+    for (rows, cols, vr, vc) in ((H, 64, H, VALID), (H, H, H, H), (H, H, H, H), (64, H, VALID, H)):
         lim = math.sqrt(6.0 / (vr + vc))
         W = np.zeros((rows, cols), dtype=np.float32)
         W[:vr, :vc] = rng.uniform(-lim, lim, size=(vr, vc))
@@ -104,6 +110,11 @@ class Surrogate:
     # ---------------------------------------------------------------- model state
     def set_model(self, params, norm, ostat, meta=None):
         import torch
+        n = int(self.L.kx_mlp_n_params())
+        if np.asarray(params).size != n or np.asarray(norm).size != 4 * IN or np.asarray(ostat).size != 2 * IN:
+            raise ValueError(f'surrogate model shape: {np.asarray(params).size} parameters (this library: '
+                             f'{n}, hidden width {hidden_width()}), {np.asarray(norm).size} input and '
+                             f'{np.asarray(ostat).size} output statistics (expected {4 * IN}, {2 * IN})')
         self.params = torch.as_tensor(np.asarray(params, np.float32), device=self.dev).contiguous()
         self.norm = torch.as_tensor(np.asarray(norm, np.float32), device=self.dev).contiguous()
         self.ostat = torch.as_tensor(np.asarray(ostat, np.float32), device=self.dev).contiguous()
@@ -143,7 +154,7 @@ class Surrogate:
         return tables.load(self.kx.dir)[1]
 
     # ---------------------------------------------------------------- training
-    def train(self, n_train=1 << 24, epochs=20, batch=8192, lr=2e-3, seed=st.BASE_SEED + 4,
+    def train(self, n_train=1 << 24, epochs=60, batch=8192, lr=2e-3, seed=st.BASE_SEED + 4,
               log=print, holdout=65536, hull_margin=0.02, near_eq_frac=0.1):
         import torch
         N, dev = self.N, self.dev
@@ -202,8 +213,9 @@ class Surrogate:
         P = params.numel()
         grads = torch.zeros(P, dtype=torch.float32, device=dev)
         m1, m2 = torch.zeros_like(grads), torch.zeros_like(grads)
-        acts = torch.empty(6 * batch * 256 + batch * IN, dtype=torch.float32, device=dev)
-        work = torch.empty(2 * batch * 256, dtype=torch.float32, device=dev)
+        H = hidden_width()
+        acts = torch.empty(6 * batch * H + batch * IN, dtype=torch.float32, device=dev)
+        work = torch.empty(2 * batch * H, dtype=torch.float32, device=dev)
         part = torch.empty((batch * IN + 255) // 256, dtype=torch.float32, device=dev)
         steps_per_epoch = n_train // batch
         total = epochs * steps_per_epoch
@@ -231,7 +243,7 @@ class Surrogate:
             l_ = float(ep_loss) / (steps_per_epoch * batch * VALID)
             hist.append(l_)
             log(f'surrogate: epoch {ep + 1}/{epochs} mse {l_:.3e} ({time.time() - t0:.1f} s)')
-        meta = {'n_train': n_train, 'epochs': epochs, 'batch': batch, 'lr': lr, 'seed': seed,
+        meta = {'hidden': hidden_width(), 'n_train': n_train, 'epochs': epochs, 'batch': batch, 'lr': lr, 'seed': seed,
                 'near_eq_frac': near_eq_frac,
                 'loss_history': hist, 'train_seconds': time.time() - t0,
                 'mechanism': os.path.basename(self.kx.dir)}
@@ -297,7 +309,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description='train the transport surrogate on the GPU')
     ap.add_argument('--mech', default='gri30')
     ap.add_argument('--n-train', type=int, default=1 << 24)
-    ap.add_argument('--epochs', type=int, default=20)
+    ap.add_argument('--epochs', type=int, default=60)
     ap.add_argument('--batch', type=int, default=8192)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--out', default=None)

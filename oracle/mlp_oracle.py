@@ -1,9 +1,9 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy restatement of the surrogate MLP (forward, backward of
 the masked MSE, one Adam step), the reference the tests hold kinetix_amd/csrc/kx_mlp.hip to.
 
-Layer layout and parameter block as in kx_mlp.hip: W1[256][64] b1 W2[256][256] b2 W3 b3
-W4[64][256] b4; GELU in its tanh form; loss = mean over the batch and the first 55 outputs of
-(Y - T)^2.
+Layer layout and parameter block as in kx_mlp.hip: W1[H][64] b1 W2[H][H] b2 W3 b3 W4[64][H]
+b4 with H = HIDDEN = 128; GELU in its tanh form; loss = mean over the batch and the first 55
+outputs of (Y - T)^2.
 
 Pinned by tests/test_mlp_oracle.py, independently of this code: the forward pass and GELU
 against torch's float64 operators, the gradient against fourth-order finite differences of the
@@ -12,7 +12,8 @@ and adam_step against the closed form of Adam's first two steps.
 """
 import numpy as np
 
-SHAPES = ((256, 64), (256, 256), (256, 256), (64, 256))
+HIDDEN = 128
+SHAPES = ((HIDDEN, 64), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (64, HIDDEN))
 VALID = 55
 
 

@@ -9,7 +9,7 @@ import torch
 
 from kinetix_amd import states
 from kinetix_amd.api import Kinetix
-from kinetix_amd.surrogate import IN, Surrogate, _bind, _p, init_params
+from kinetix_amd.surrogate import IN, Surrogate, _bind, _p, hidden_width, init_params
 from oracle import mlp_oracle as mo
 
 pytestmark = pytest.mark.gpu
@@ -59,15 +59,16 @@ def test_forward_and_gradients_vs_oracle(L, case):
     p, X, T = case
     B = X.shape[0]
     dp, dX, dT = t32(p), t32(X), t32(T)
-    acts = torch.empty(6 * B * 256 + B * IN, dtype=torch.float32, device=dev)
+    H = hidden_width()
+    acts = torch.empty(6 * B * H + B * IN, dtype=torch.float32, device=dev)
     grads = torch.empty_like(dp)
-    work = torch.empty(2 * B * 256, dtype=torch.float32, device=dev)
+    work = torch.empty(2 * B * H, dtype=torch.float32, device=dev)
     part = torch.empty((B * IN + 255) // 256, dtype=torch.float32, device=dev)
     assert L.kx_mlp_forward(_p(dp), B, _p(dX), _p(acts), None) == 0
     assert L.kx_mlp_backward(_p(dp), B, _p(dX), _p(acts), _p(dT), _p(grads), _p(work), _p(part), None) == 0
     torch.cuda.synchronize()
     loss, Y, g = mo.loss_and_grad(p, X, T)
-    Yg = acts[6 * B * 256:].view(B, IN).cpu().numpy()
+    Yg = acts[6 * B * H:].view(B, IN).cpu().numpy()
     assert np.max(np.abs(Yg - Y)) <= 1e-5 * np.max(np.abs(Y))
     gg = grads.cpu().numpy()
     assert np.max(np.abs(gg - g)) <= 1e-4 * np.max(np.abs(g))

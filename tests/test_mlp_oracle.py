@@ -120,7 +120,7 @@ def test_gradient_vs_torch_autograd():
         blk = slice(o, o + n)
         assert np.max(np.abs(g[blk] - ref[blk])) <= 1e-12 * np.max(np.abs(ref[blk])), name
     # outputs past the first 55 take no gradient: the last layer's rows 55..63 stay zero
-    W4 = slice(BLOCKS[6][1] + mo.VALID * 256, BLOCKS[6][1] + 64 * 256)
+    W4 = slice(BLOCKS[6][1] + mo.VALID * mo.HIDDEN, BLOCKS[6][1] + 64 * mo.HIDDEN)
     assert np.all(g[W4] == 0.0)
 
 
