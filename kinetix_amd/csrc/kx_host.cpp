@@ -46,6 +46,7 @@ struct Mech {
   hipModule_t module = nullptr;
   std::map<std::string, hipFunction_t> fn;
   std::map<std::string, unsigned> block;     // each kernel's launch bound (threads per block)
+  std::map<std::string, int> lanes;          // lanes per state of kernels that share a state
   int queue_lanes = 1;                        // lanes per queued state of kx_bk2_fp64_queue
   long long queue_grid_cap = 0;               // its workgroups resident at once (strided form)
   int n_species = 0, n_active = 0, n_reactions = 0;
@@ -130,6 +131,8 @@ int launch(Mech* m, const char* name, long long n_lanes, void* stream, void** ar
   if (n_lanes == 0) return KX_OK;
   KX_HIP(hipSetDevice(m->device));
   const unsigned block = m->block.at(name);
+  auto l = m->lanes.find(name);
+  if (l != m->lanes.end()) n_lanes *= l->second;
   const long long g = (n_lanes + (long long)block - 1) / block;
   if (g > 0x7fffffffLL) return fail(KX_ERR_INVALID_ARG, "grid too large");
   const unsigned grid = (unsigned)g;
@@ -193,6 +196,23 @@ static int finish_init(Mech* m, kx_mech_t* out)
     }
     m->fn[k] = f;
     m->block[k] = (unsigned)bound;
+    // a kernel that spreads each state over several lanes exports how many
+    hipDeviceptr_t lp = nullptr;
+    size_t lbytes = 0;
+    if (hipModuleGetGlobal(&lp, &lbytes, m->module, (std::string(k) + "_lanes_per_state").c_str()) ==
+            hipSuccess &&
+        lbytes == sizeof(int)) {
+      int lanes = 0;
+      if (hipMemcpyDtoH(&lanes, lp, sizeof(int)) != hipSuccess || lanes < 1 || bound % lanes) {
+        (void)hipGetLastError();
+        (void)hipModuleUnload(m->module);
+        delete m;
+        return fail(KX_ERR_HIP, std::string("unusable lanes per state for ") + k);
+      }
+      m->lanes[k] = lanes;
+    } else {
+      (void)hipGetLastError();
+    }
   }
   // the queue kernel may share each state among several lanes; it then exports how many
   {

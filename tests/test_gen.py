@@ -377,3 +377,56 @@ def test_cooperative_queue_kernel_arithmetic_vs_oracle(gri, oracle):
     t2 = dict(t)
     t2['C1'] = np.asarray(t['C1']).T.copy()
     assert np.max(np.abs(_queue_kernel_numpy(t2, s, p)[1] / ref[1] - 1)) > 1e-6
+
+
+def _tile_kernel_numpy(t, s, p, wil=None):
+    """The symmetric tile transport kernel (kinetix_amd/gen/emit_bk2_tile.py) in numpy, from the
+    tables it reads: the upper-triangle pair fits at base(k) + j, and the [k][j] (C1, C2) records."""
+    from kinetix_amd.gen import emit_bk2_tile as BT
+    N = int(t['n_species'])
+    fit = BT.upper_fits(t)
+    wil = np.stack([np.asarray(t['C1']), np.asarray(t['C2'])], axis=-1) if wil is None else wil
+    T = s[0]
+    lnT = np.log(T)
+    P = np.stack([np.ones_like(lnT), lnT, lnT ** 2, lnT ** 3, lnT ** 4])
+    x = np.maximum(0.0, s[1:]) * np.asarray(t['rcpW'])[:, None]
+    Mbar = 1.0 / x.sum(0)
+    x = x * Mbar
+    sums = np.zeros_like(x)
+    for k in range(N):
+        base = k * (N - 1) - k * (k - 1) // 2 - k - 1
+        for j in range(k + 1, N):
+            rD = 1.0 / (fit[base + j] @ P)
+            sums[k] += x[j] * rD
+            sums[j] += x[k] * rD
+    Dkm = T * np.sqrt(T) * (Mbar - np.asarray(t['W'])[:, None] * x) / (p * Mbar * sums)
+    rhoD = p / float(t['R']) / T * Mbar * Dkm
+    e = np.asarray(t['visc']) @ P
+    Sw = np.zeros_like(x)
+    for k in range(N):
+        f = wil[k, :, 0][:, None] + wil[k, :, 1][:, None] * e[k] / e
+        Sw[k] = (x * f * f).sum(0)
+    mu = np.sqrt(T) * (x * e * e / Sw).sum(0)
+    lam_k = np.asarray(t['cond']) @ P
+    cond = np.sqrt(T) * 0.5 * ((x * lam_k).sum(0) + 1.0 / (x / lam_k).sum(0))
+    return cond, mu, rhoD, Dkm
+
+
+def test_tile_transport_tables_vs_oracle(gri, oracle):
+    """The tile kernel's pair-index arithmetic and Wilke record orientation against the oracle's
+    FP64 transport (per-state pressure); a transposed Wilke record fails by far more."""
+    from kinetix_amd import states
+    from kinetix_amd.gen import emit_bk2_tile as BT
+    info, t = gri
+    s, p = states.sample_transport_set(48, info['species'], t['W'], seed=22)
+    got = _tile_kernel_numpy(t, s, p)
+    ref = oracle.transport(s, 1.0, 1.0, p_state=p)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g / r - 1)) < 1e-12
+    wil_t = np.stack([np.asarray(t['C1']).T, np.asarray(t['C2']).T], axis=-1)
+    assert np.max(np.abs(_tile_kernel_numpy(t, s, p, wil_t)[1] / ref[1] - 1)) > 1e-6
+    # the index formula enumerates the upper triangle row by row, at 160 species too
+    for N in (53, 160):
+        seen = [k * (N - 1) - k * (k - 1) // 2 - k - 1 + j for k in range(N) for j in range(k + 1, N)]
+        assert seen == list(range(N * (N - 1) // 2))
+    assert BT.lds_bytes({'n_species': 160}) == 160 * 1024
