@@ -23,6 +23,8 @@ def main(argv=None):
     ap.add_argument('--tables-only', action='store_true', help='write tables, no kernels')
     ap.add_argument('--rolled', action='store_true',
                     help='loop forms of the FP64/mixed kernels (default above 96 species)')
+    ap.add_argument('--bk2-loop', action='store_true',
+                    help='with the loop forms: transport one state per lane instead of state tiles')
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -36,7 +38,12 @@ def main(argv=None):
     counts.write(args.output, tab)
     if not args.tables_only:
         from . import emit_hip
-        emit_hip.write(args.output, mech, tab, {'rolled': True} if args.rolled else None)
+        opts = {}
+        if args.rolled:
+            opts['rolled'] = True
+        if args.bk2_loop:
+            opts['bk2_loop'] = True
+        emit_hip.write(args.output, mech, tab, opts or None)
     print(f'{mech.name}: {mech.n_species} species ({mech.n_active} active), '
           f'{mech.n_reactions} reactions -> {args.output} [{time.time() - t0:.1f}s]')
     return 0

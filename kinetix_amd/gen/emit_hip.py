@@ -1085,17 +1085,12 @@ def generate(tables, info, opts=None):
         parts.append(emit_coop.emit_bk1(tables, info, 'fp64', o['coop_lps']))
         parts.append(emit_coop.emit_bk1(tables, info, 'mixed_valu', o['coop_lps']))
         if 'cond' in tables:
-            # transport over state tiles: species rows across wavefronts, x and 1/eta in LDS
-            # (64-state tiles up to 160 species; 32-state tiles, both rows per pair, up to 320;
-            # beyond that the one-lane loop form)
+            # transport over 64-state tiles (species rows across wavefronts, x and 1/eta in
+            # LDS) up to 160 species; beyond that, or asked for (bk2_loop), the one-lane loop form
             from . import emit_bk2_tile as BT
+            tile = BT.lds_bytes(tables) <= 160 * 1024 and not o.get('bk2_loop')
             for ps in (False, True):
-                if BT.lds_bytes(tables) <= 160 * 1024:
-                    parts.append(BT.emit(tables, info, ps))
-                elif BT.lds_bytes(tables, 32, 512) <= 160 * 1024:
-                    parts.append(BT.emit(tables, info, ps, 4, 32, 512, False))
-                else:
-                    parts.append(emit_rolled.emit_bk2(tables, info, ps))
+                parts.append(BT.emit(tables, info, ps) if tile else emit_rolled.emit_bk2(tables, info, ps))
         parts.append(_with_pool('kx_thermo_fp64_k', _emit_thermo_fp64, tables, info))
         return '\n\n'.join(parts) + '\n'
     order = None

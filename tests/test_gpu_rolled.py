@@ -104,8 +104,8 @@ def test_fp64_transport_vs_oracle(setup):
 
 def test_full_size_consistency(setup):
     """At the synthetic workload's full size (2,097,152 states): every output finite, mixed
-    within 1e-5 relative L2 per species of FP64 over all states, and a sub-view of the SoA
-    reproducing the full run bit for bit."""
+    within 1e-5 relative L2 per species of FP64 over all states, transport against the oracle
+    on a sample, and a sub-view of the SoA reproducing the full production-rate run bit for bit."""
     name, info, t, kx, orc = setup
     if name != 'synth160':
         pytest.skip('full-size run for the synthetic workload only')
@@ -125,6 +125,20 @@ def test_full_size_consistency(setup):
     den = torch.linalg.vector_norm(F[1:], dim=1)
     rel = (num[den > 0] / den[den > 0]).max().item()
     assert rel < 1e-5, rel
+    # transport at full size: finite everywhere, a sample against the oracle
+    v = torch.empty(n, dtype=torch.float64, device='cuda')
+    c, r = torch.empty_like(v), torch.empty(N * n, dtype=torch.float64, device='cuda')
+    kx.mixtureAvgTransportProps(n, n, n, 1.0, ds, v, c, r)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(v).all() and torch.isfinite(c).all() and torch.isfinite(r).all())
+    idx = np.random.default_rng(5).choice(n, 256, replace=False)
+    ii = torch.from_numpy(idx).cuda()
+    s_sub = ds.view(N + 1, n)[:, ii].cpu().numpy()
+    c0, v0, r0, _ = orc.transport(s_sub, 1.0, threads=8)
+    assert cond_err(v[ii].cpu().numpy(), v0) < 1e-12
+    assert cond_err(c[ii].cpu().numpy(), c0) < 1e-12
+    assert cond_err(r.view(N, n)[:, ii].cpu().numpy(), r0) < 1e-12
+    del v, c, r
     lo, cnt = 123457, 100000
     sub = torch.full_like(ds, np.nan)
     kx.productionRates(cnt, n, n, 1.0, ds[lo:], sub[lo:])
