@@ -39,25 +39,9 @@ HALVES = 16                  # wavefronts per workgroup: the row classes
 BLOCK = TS * HALVES          # 1024 lanes
 
 
-def pair_table(t):
-    """[N][N][8]: D'_kj fit (5; the diagonal is zero and never used), C1_kj, C2_kj, 0 (the
-    records of the form with both rows per pair, symmetric=False)."""
+def lds_bytes(t):
     N = int(t['n_species'])
-    d = np.asarray(t['diff'], dtype=np.float64)
-    tab = np.zeros((N, N, 8))
-    for k in range(N):
-        for j in range(N):
-            if j != k:
-                tab[k, j, :5] = d[max(k, j), min(k, j)]
-    tab[:, :, 5] = np.asarray(t['C1'])
-    tab[:, :, 6] = np.asarray(t['C2'])
-    return tab
-
-
-def lds_bytes(t, ts=TS, block=BLOCK):
-    N = int(t['n_species'])
-    rows = block // ts
-    return 8 * (2 * N * ts + (0 if N >= 3 * rows else 3 * rows * ts))
+    return 8 * (2 * N * TS + (0 if N >= 3 * HALVES else 3 * HALVES * TS))
 
 
 def _poly(c, k):
@@ -73,12 +57,8 @@ def upper_fits(t):
     return np.array([d[j, k] for k in range(N) for j in range(k + 1, N)]).reshape(-1, 5)
 
 
-def emit(t, info, pstate=False, unroll=4, ts=TS, block=BLOCK, symmetric=True):
+def emit(t, info, pstate=False, unroll=4):
     N = int(t['n_species'])
-    TS, HALVES = ts, block // ts              # states per tile, row classes
-    BLOCK = block
-    if symmetric and TS != 64:
-        raise ValueError('the symmetric diffusion phase takes one row per wavefront (64-state tiles)')
     rcp_diff = bool(int(t.get('rcp_diff', 0)))
     name = 'kx_bk2_fp64_pstate' if pstate else 'kx_bk2_fp64'
     pre = f'{name}_t'
@@ -87,11 +67,8 @@ def emit(t, info, pstate=False, unroll=4, ts=TS, block=BLOCK, symmetric=True):
     def table(nm, arr):
         a = np.asarray(arr, dtype=np.float64).reshape(-1)
         S(f'__device__ const double {nm}[{a.size}] = {{' + ', '.join(repr(float(v)) for v in a) + '};', 0)
-    if symmetric:
-        table(f'{pre}_fit', upper_fits(t))
-        table(f'{pre}_wil', np.stack([np.asarray(t['C1']), np.asarray(t['C2'])], axis=-1))
-    else:
-        table(f'{pre}_pair', pair_table(t))
+    table(f'{pre}_fit', upper_fits(t))
+    table(f'{pre}_wil', np.stack([np.asarray(t['C1']), np.asarray(t['C2'])], axis=-1))
     table(f'{pre}_visc', t['visc'])
     table(f'{pre}_cond', t['cond'])
     table(f'{pre}_rcpW', t['rcpW'])
@@ -113,11 +90,8 @@ def emit(t, info, pstate=False, unroll=4, ts=TS, block=BLOCK, symmetric=True):
     alias = N >= 3 * HALVES                     # the three partial sums fit over the x rows
     if not alias:
         S(f'__shared__ double red[3 * {HALVES} * {TS}];')
-    if TS == 64:
-        S('const int s = threadIdx.x & 63;')
-        S('const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // row class: one per wave')
-    else:
-        S(f'const int s = threadIdx.x & {TS - 1}, h = threadIdx.x / {TS};   // state in the tile, row class')
+    S('const int s = threadIdx.x & 63;')
+    S('const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // row class: one per wave')
     S('const long long id0 = t0 + s;')
     S('const bool live = id0 < n_eff;')
     S('const long long id = live ? id0 : t0;              // lanes past the end redo the tile\'s first state, store nothing')
@@ -143,109 +117,67 @@ def emit(t, info, pstate=False, unroll=4, ts=TS, block=BLOCK, symmetric=True):
     S(f'for (int v = 0; v < {HALVES}; ++v) ysum += sre[v * {TS} + s];')
     S('const double Mbar = 1.0 / ysum;')
     S('__syncthreads();                                   // partials read before 1/eta overwrites them')
-    if symmetric:
-        NI = 2 * ((N + 2 * HALVES - 1) // (2 * HALVES))
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
-        S(f'  sx[k * {TS} + s] *= Mbar;')
-        S(f'  sre[k * {TS} + s] = 0.0;                     // the diffusion row sums')
-        S('}')
-        S('__syncthreads();')
-        S('const double TsqrT = T * sqrT;')
-        S(f'const double rho = p * {E._literal(1.0 / float(t["R"]))} * rcpT * Mbar;')
-        S('const double pMbar = p * Mbar;')
-        S('// diffusion, each pair once: the wavefront of row k takes the pairs j > k and adds')
-        S('// x_k / D_kj to row j with an LDS atomic (rows k and 31 - k of each block of 32 share a')
-        S('// wavefront, so every wavefront has about the same number of pairs)')
-        S('#pragma unroll 1')
-        S(f'for (int i = 0; i < {NI}; ++i) {{')
-        S(f'  const int k = {2 * HALVES} * (i >> 1) + ((i & 1) ? {2 * HALVES - 1} - h : h);')
-        S(f'  if (k >= {N}) continue;')
-        S(f'  const double xk = sx[k * {TS} + s];')
-        S(f'  const int base = k * {N - 1} - k * (k - 1) / 2 - k - 1;   // pair (k, j) at base + j')
-        S('  double sk = 0.0;')
-        S(f'  #pragma unroll {unroll}')
-        S(f'  for (int j = k + 1; j < {N}; ++j) {{')
-        S(f'    const double* c = {pre}_fit + 5 * (base + j);')
-        poly = 'c[0] + c[1] * lnT + c[2] * lnT2 + c[3] * lnT3 + c[4] * lnT4'
-        S(f'    const double rD = {poly if rcp_diff else "kx_rcp(" + poly + ")"};')
-        S(f'    sk += sx[j * {TS} + s] * rD;')
-        S(f'    atomicAdd(sre + j * {TS} + s, xk * rD);')
-        S('  }')
-        S(f'  atomicAdd(sre + k * {TS} + s, sk);')
-        S('}')
-        S('__syncthreads();')
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
-        S(f'  const double D = TsqrT * (Mbar - {pre}_W[k] * sx[k * {TS} + s]) / (pMbar * sre[k * {TS} + s]);')
-        S('  if (live) {')
-        S('    rhoD[k * offset + id] = rho * D;')
-        if pstate:
-            S('    Dkm[k * offset + id] = D;')
-        S('  }')
-        S('}')
-        S('__syncthreads();                                   // row sums read before 1/eta overwrites them')
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) sre[k * {TS} + s] = kx_rcp({_poly(pre + "_visc", "k")});')
-        S('__syncthreads();')
-        S('double vpart = 0.0, s1 = 0.0, s2 = 0.0;')
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
-        S(f'  const double xk = sx[k * {TS} + s];')
-        S(f'  const double e = {_poly(pre + "_visc", "k")};')
-        S(f'  const double* __restrict__ pw = {pre}_wil + k * {2 * N};')
-        S('  double sw = 0.0;')
-        S(f'  #pragma unroll {2 * unroll}')
-        S(f'  for (int j = 0; j < {N}; ++j) {{')
-        S(f'    const double f = pw[2 * j] + pw[2 * j + 1] * e * sre[j * {TS} + s];')
-        S(f'    sw += sx[j * {TS} + s] * f * f;')
-        S('  }')
-        S('  vpart += xk * e * e / sw;')
-        S(f'  const double l = {_poly(pre + "_cond", "k")};')
-        S('  s1 += xk * l;')
-        S('  s2 += xk / l;')
-        S('}')
-    else:
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
-        S(f'  sx[k * {TS} + s] *= Mbar;')
-        S(f'  sre[k * {TS} + s] = kx_rcp({_poly(pre + "_visc", "k")});')
-        S('}')
-        S('__syncthreads();')
-        S('const double TsqrT = T * sqrT;')
-        S(f'const double rho = p * {E._literal(1.0 / float(t["R"]))} * rcpT * Mbar;')
-        S('const double pMbar = p * Mbar;')
-        S('double vpart = 0.0, s1 = 0.0, s2 = 0.0;')
-        S('#pragma unroll 1')
-        S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
-        S(f'  const double xk = sx[k * {TS} + s];')
-        S(f'  const double e = {_poly(pre + "_visc", "k")};')
-        S(f'  const double* __restrict__ pc = {pre}_pair + k * {8 * N};')
-        S('  double sd = 0.0, sw = 0.0;')
-        S(f'  #pragma unroll {unroll}')
-        S(f'  for (int j = 0; j < {N}; ++j) {{')
-        S('    const double* c = pc + 8 * j;')
-        S(f'    const double xj = sx[j * {TS} + s];')
-        S(f'    const double rj = sre[j * {TS} + s];')
-        S('    const double pd = c[0] + c[1] * lnT + c[2] * lnT2 + c[3] * lnT3 + c[4] * lnT4;')
-        if rcp_diff:
-            S('    sd += xj * (j == k ? 0.0 : pd);')
-        else:
-            S('    sd += xj * (j == k ? 0.0 : kx_rcp(pd));          // the diagonal fit is zero: never inverted into the sum')
-        S('    const double f = c[5] + c[6] * e * rj;')
-        S('    sw += xj * f * f;')
-        S('  }')
-        S(f'  const double D = TsqrT * (Mbar - {pre}_W[k] * xk) / (pMbar * sd);')
-        S('  if (live) {')
-        S('    rhoD[k * offset + id] = rho * D;')
-        if pstate:
-            S('    Dkm[k * offset + id] = D;')
-        S('  }')
-        S('  vpart += xk * e * e / sw;')
-        S(f'  const double l = {_poly(pre + "_cond", "k")};')
-        S('  s1 += xk * l;')
-        S('  s2 += xk / l;')
-        S('}')
+    NI = 2 * ((N + 2 * HALVES - 1) // (2 * HALVES))
+    S('#pragma unroll 1')
+    S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
+    S(f'  sx[k * {TS} + s] *= Mbar;')
+    S(f'  sre[k * {TS} + s] = 0.0;                     // the diffusion row sums')
+    S('}')
+    S('__syncthreads();')
+    S('const double TsqrT = T * sqrT;')
+    S(f'const double rho = p * {E._literal(1.0 / float(t["R"]))} * rcpT * Mbar;')
+    S('const double pMbar = p * Mbar;')
+    S('// diffusion, each pair once: the wavefront of row k takes the pairs j > k and adds')
+    S('// x_k / D_kj to row j with an LDS atomic (rows k and 31 - k of each block of 32 share a')
+    S('// wavefront, so every wavefront has about the same number of pairs)')
+    S('#pragma unroll 1')
+    S(f'for (int i = 0; i < {NI}; ++i) {{')
+    S(f'  const int k = {2 * HALVES} * (i >> 1) + ((i & 1) ? {2 * HALVES - 1} - h : h);')
+    S(f'  if (k >= {N}) continue;')
+    S(f'  const double xk = sx[k * {TS} + s];')
+    S(f'  const int base = k * {N - 1} - k * (k - 1) / 2 - k - 1;   // pair (k, j) at base + j')
+    S('  double sk = 0.0;')
+    S(f'  #pragma unroll {unroll}')
+    S(f'  for (int j = k + 1; j < {N}; ++j) {{')
+    S(f'    const double* c = {pre}_fit + 5 * (base + j);')
+    poly = 'c[0] + c[1] * lnT + c[2] * lnT2 + c[3] * lnT3 + c[4] * lnT4'
+    S(f'    const double rD = {poly if rcp_diff else "kx_rcp(" + poly + ")"};')
+    S(f'    sk += sx[j * {TS} + s] * rD;')
+    S(f'    atomicAdd(sre + j * {TS} + s, xk * rD);')
+    S('  }')
+    S(f'  atomicAdd(sre + k * {TS} + s, sk);')
+    S('}')
+    S('__syncthreads();')
+    S('#pragma unroll 1')
+    S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
+    S(f'  const double D = TsqrT * (Mbar - {pre}_W[k] * sx[k * {TS} + s]) / (pMbar * sre[k * {TS} + s]);')
+    S('  if (live) {')
+    S('    rhoD[k * offset + id] = rho * D;')
+    if pstate:
+        S('    Dkm[k * offset + id] = D;')
+    S('  }')
+    S('}')
+    S('__syncthreads();                                   // row sums read before 1/eta overwrites them')
+    S('#pragma unroll 1')
+    S(f'for (int k = h; k < {N}; k += {HALVES}) sre[k * {TS} + s] = kx_rcp({_poly(pre + "_visc", "k")});')
+    S('__syncthreads();')
+    S('double vpart = 0.0, s1 = 0.0, s2 = 0.0;')
+    S('#pragma unroll 1')
+    S(f'for (int k = h; k < {N}; k += {HALVES}) {{')
+    S(f'  const double xk = sx[k * {TS} + s];')
+    S(f'  const double e = {_poly(pre + "_visc", "k")};')
+    S(f'  const double* __restrict__ pw = {pre}_wil + k * {2 * N};')
+    S('  double sw = 0.0;')
+    S(f'  #pragma unroll {2 * unroll}')
+    S(f'  for (int j = 0; j < {N}; ++j) {{')
+    S(f'    const double f = pw[2 * j] + pw[2 * j + 1] * e * sre[j * {TS} + s];')
+    S(f'    sw += sx[j * {TS} + s] * f * f;')
+    S('  }')
+    S('  vpart += xk * e * e / sw;')
+    S(f'  const double l = {_poly(pre + "_cond", "k")};')
+    S('  s1 += xk * l;')
+    S('  s2 += xk / l;')
+    S('}')
     S('__syncthreads();                                   // every row done with x and 1/eta')
     if alias:
         S(f'double* const red = sx;                            // [3][{HALVES}][{TS}] over the x rows')
