@@ -4,7 +4,7 @@
  *
  * Model: y = b(T, p, Y) + out_mean + out_std * MLP((x - in_mean) / in_std), with
  * y = (ln mu, ln lambda, ln D_1m .. ln D_Nm), x = (normalised T, normalised ln p, Y_1..Y_N)
- * padded to 64, b an FP32 physics baseline, MLP 64 -> 3x128 GELU -> 64. All buffers are HIP
+ * padded to 64, b an FP32 physics baseline, MLP 64 -> 3x256 GELU -> 64. All buffers are HIP
  * device pointers; all calls are asynchronous on `stream` (NULL = default stream).
  */
 #ifndef KINETIX_MI_SURROGATE_H
@@ -118,7 +118,7 @@ int kx_sur_transport_noalloc(kx_mech_t m, int n_states, int offsetT, int offset,
 
 /* ---- mechanism-independent MLP kernels ---- */
 
-int kx_mlp_width(void);             /* hidden width H (128) */
+int kx_mlp_width(void);             /* hidden width H (256) */
 int kx_mlp_n_params(void);          /* FP32 parameter block length */
 int kx_mlp_n_weights_bf16(void);    /* BF16 weight block length */
 

@@ -1,4 +1,4 @@
-// Hand-written kernels of the transport-surrogate MLP: 64 -> kH -> kH -> kH -> 64 (kH = 128)
+// Hand-written kernels of the transport-surrogate MLP: 64 -> kH -> kH -> kH -> 64 (kH = 256)
 // (55 inputs / 55 outputs zero-padded to 64), GELU (tanh form) on the hidden layers.
 //
 // Training (FP32 master weights): tiled FP32 GEMM with either operand transposed, bias +

@@ -25,11 +25,10 @@
 
 namespace kxmlp {
 
-// Hidden width 128: the inference kernel's time is ~70 % MLP (its activation VALU work and
-// MFMA chains), so the width sets it -- 2.65 ms at 256, 2.13 at 192, 1.62 at 128 for 4M
-// states -- while the outputs' error is set by the BF16 rounding of the activations, not by
-// the width. A multiple of 64 (the training GEMM tiles).
-constexpr int kIn = 64, kH = 128, kOut = 64, kValidOut = 55;
+// Hidden width: 256, the transport surrogate's specified architecture (55 -> 3 x 256 GELU ->
+// 55). The code takes any multiple of 64 (the training GEMM tiles); the fused inference
+// kernel's time follows the width (2.65 ms at 256, 2.13 at 192, 1.62 at 128 for 4M states).
+constexpr int kIn = 64, kH = 256, kOut = 64, kValidOut = 55;
 static_assert(kH % 64 == 0, "training GEMM tiles are 64 wide");
 // FP32 parameter block: W1[kH][64] b1[kH] W2[kH][kH] b2[kH] W3[kH][kH] b3[kH] W4[64][kH] b4[64]
 constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH;
@@ -80,7 +79,7 @@ __device__ __forceinline__ float gelu_fast(float x)
 
 // activations of one pass (SB blocks of 16 states) for one layer: [k block][state block]
 // fragments, kH / 32 x SB x 4 registers per layer. A pass of SB = 4 covers the wave's 64
-// states (input + output layer: 2 kH / 8 x 4 registers, 128 at kH = 128); SB = 2 covers 32
+// states (input + output layer: 2 kH / 8 x 4 registers, 256 at kH = 256); SB = 2 covers 32
 // states with half that, at twice the weight reads per state.
 template <int KB, int SB = 4>
 using Act = u32x4[KB][SB];

@@ -2,7 +2,7 @@
 
 The surrogate predicts the mixture-averaged transport outputs y = (ln mu, ln lambda,
 ln D_1m .. ln D_Nm) as  y = b(T, p, Y) + mean + std * MLP(x),  where b is a cheap FP32
-physics baseline and the MLP (55 -> 3x128 GELU -> 55, BF16 MFMA inference; hand-written
+physics baseline and the MLP (55 -> 3x256 GELU -> 55, BF16 MFMA inference; hand-written
 kernels in kinetix_amd/csrc/kx_mlp.hip) learns the standardised residual:
 
   * mu baseline: the Wilke mixing rule (the formula the reference uses,
@@ -80,7 +80,7 @@ def baseline_numpy(t, species, T, p, Y):
 XS = 68            # LDS row stride (bf16) of the staged MLP inputs in the fused kernel
 
 
-def emit(t, info, passes=1):
+def emit(t, info, passes=2):
     N = int(t['n_species'])
     if N + 2 > IN:
         return ''                                   # surrogate width is fixed at 64
@@ -245,9 +245,9 @@ def emit(t, info, passes=1):
     S(f'__device__ const float kx_sur_dpoly[{5 * N}] = {{' +
       ', '.join(f'{float(v)!r}f' for v in dtab.reshape(-1)) + '};', 0)
     S(f'// One state per lane for the physics part; the MLP (kx_mlp_wave.h) in {passes} pass(es) of', 0)
-    S(f'// {64 // passes} states per wavefront (at hidden width 128 one pass of 64: 1.53 ms for 4M states', 0)
-    S('// against 1.64 in two passes of 32, which halve the activation registers but read every', 0)
-    S('// weight fragment twice); two waves share a SIMD (one wave\'s MFMAs overlap the other\'s', 0)
+    S(f'// {64 // passes} states per wavefront (at hidden width 256 two passes of 32: the activations take', 0)
+    S('// 128 VGPRs instead of 256 -- 2.81 ms for 4M states against 3.74 in one pass; at width 128', 0)
+    S('// one pass of 64 wins, 1.53 against 1.64 ms); two waves share a SIMD (one wave\'s MFMAs overlap the other\'s', 0)
     S('// feature and GELU VALU work). LDS carries the inputs', 0)
     S('// and four floats per state (ln mu and ln lambda baselines + output means, ln T,', 0)
     S('// 1.5 ln T - ln p) between the two layouts; the diffusion baselines are recomputed per', 0)

@@ -2,7 +2,7 @@
 the masked MSE, one Adam step), the reference the tests hold kinetix_amd/csrc/kx_mlp.hip to.
 
 Layer layout and parameter block as in kx_mlp.hip: W1[H][64] b1 W2[H][H] b2 W3 b3 W4[64][H]
-b4 with H = HIDDEN = 128; GELU in its tanh form; loss = mean over the batch and the first 55
+b4 with H = HIDDEN = 256; GELU in its tanh form; loss = mean over the batch and the first 55
 outputs of (Y - T)^2.
 
 Pinned by tests/test_mlp_oracle.py, independently of this code: the forward pass and GELU
@@ -12,7 +12,7 @@ and adam_step against the closed form of Adam's first two steps.
 """
 import numpy as np
 
-HIDDEN = 128
+HIDDEN = 256
 SHAPES = ((HIDDEN, 64), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (64, HIDDEN))
 VALID = 55
 

@@ -45,8 +45,8 @@ def test_every_declared_symbol_is_exported(L):
 
 def test_host_only_queries(L):
     H = L.kx_mlp_width()
-    assert H == 128
-    assert L.kx_mlp_n_params() == H * 64 + H + 2 * (H * H + H) + 64 * H + 64 == 49600
+    assert H == 256
+    assert L.kx_mlp_n_params() == H * 64 + H + 2 * (H * H + H) + 64 * H + 64 == 164672
     assert L.kx_mlp_n_weights_bf16() == (H // 16) * 2 * 512 + 2 * (H // 16) * (H // 32) * 512 \
         + 4 * (H // 32) * 512
     L.kx_sur_workspace_bytes.restype = C.c_size_t
