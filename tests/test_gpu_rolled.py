@@ -144,3 +144,47 @@ def test_full_size_consistency(setup):
     kx.productionRates(cnt, n, n, 1.0, ds[lo:], sub[lo:])
     torch.cuda.synchronize()
     assert torch.equal(sub.view(N + 1, n)[:, lo:lo + cnt], F[:, lo:lo + cnt])
+
+
+@pytest.mark.parametrize('n', [1, 63, 65, 1000])
+def test_tails_and_strided_views(setup, n):
+    """State counts that leave a partial workgroup (production rates: 64 states per
+    workgroup; transport: 64-state tiles) inside a strided SoA (offsetT = offset = n + 37):
+    every kernel against the oracle, and nothing written past state n of any row."""
+    name, info, t, kx, orc = setup
+    N = kx.nSpecies()
+    st = n + 37
+    # production rates on the rate workloads' distribution (T 1000-2500 K, 1 atm), transport
+    # on the transport workload's (T 300-2500 K, p 0.5-40 atm)
+    s, _ = states.sample_any(n, info['species'], t['W'], seed=states.BASE_SEED + 40 + n)
+    buf = np.full((N + 1, st), np.nan)
+    buf[:, :n] = s
+    ds = torch.from_numpy(buf).cuda().reshape(-1)
+    ref, sc = orc.production_rates(s, P_ATM, scale=True, threads=8)
+    for mixed in (False, 'valu'):
+        dr = torch.full_like(ds, np.nan)
+        kx.productionRates(n, st, st, 1.0, ds, dr, mixed=mixed)
+        torch.cuda.synchronize()
+        r = dr.cpu().numpy().reshape(N + 1, st)
+        assert np.all(np.isnan(r[:, n:])), 'written past the last state'
+        if mixed:
+            assert np.all(np.isfinite(r[:, :n]))
+            assert _l2(r[1:, :n], ref[1:]) < 1e-5
+        else:
+            assert cond_err(r[:, :n], ref, sc) < 1e-12
+    s, p = states.sample_any(n, info['species'], t['W'], T_range=(300.0, 2500.0),
+                             seed=states.BASE_SEED + 80 + n, p_range_atm=(0.5, 40.0))
+    buf[:, :n] = s
+    ds = torch.from_numpy(buf).cuda().reshape(-1)
+    z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device='cuda')
+    v, c, rr, d = z(st), z(st), z(N * st), z(N * st)
+    dp = torch.from_numpy(np.concatenate([p, np.full(st - n, np.nan)])).cuda()
+    kx.mixtureAvgTransportPropsPstate(n, st, st, dp, ds, v, c, rr, d)
+    torch.cuda.synchronize()
+    c0, v0, r0, d0 = orc.transport(s, 1.0, p_state=p, threads=8)
+    for got, want in ((v.cpu().numpy()[:n], v0), (c.cpu().numpy()[:n], c0)):
+        assert cond_err(got, want) < 1e-12
+    R, D = rr.cpu().numpy().reshape(N, st), d.cpu().numpy().reshape(N, st)
+    assert cond_err(R[:, :n], r0) < 1e-12 and cond_err(D[:, :n], d0) < 1e-12
+    assert np.all(np.isnan(R[:, n:])) and np.all(np.isnan(D[:, n:]))
+    assert np.all(np.isnan(v.cpu().numpy()[n:])) and np.all(np.isnan(c.cpu().numpy()[n:]))
