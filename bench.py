@@ -175,6 +175,12 @@ def main():
                   f'python -m kinetix_amd.surrogate or run --bk2 fp64', file=sys.stderr)
             return 2
         sur = Surrogate(kx, path)
+    sur_width = sur_epochs = sur_ntrain = None
+    if sur is not None:
+        from kinetix_amd.surrogate import hidden_width
+        sur_width = hidden_width()
+        sur_epochs, sur_ntrain = sur.meta.get('epochs'), sur.meta.get('n_train')
+    IN_FEATURES = N + 2
     sur_stats = []
     sur_calls = [0]
     from kinetix_amd.api import KinetixError
@@ -448,7 +454,10 @@ def main():
                                + (', 10% of states within 1% of chemical equilibrium' if near_eq is not None
                                   else ''),
                    'mechanism': a.mech, 'n_states': n, 'bk1': a.bk1, 'bk2': a.bk2,
-                   'n_species': N, 'n_reactions': kx.nReactions()},
+                   'n_species': N, 'n_reactions': kx.nReactions(),
+                   **({'surrogate_model': f'MLP {IN_FEATURES} -> 3x{sur_width} GELU -> {IN_FEATURES}, BF16 weights, '
+                                          f'FP32 accumulation, {sur_epochs} epochs on {sur_ntrain} plain-kernel states'}
+                      if sur is not None else {})},
         'roofline': roofline,
         'kernels': kern,
         'bk1_Mstates_s': round(n / m1 / 1e3, 2), 'bk2_Mstates_s': round(n / m2 / 1e3, 2),
