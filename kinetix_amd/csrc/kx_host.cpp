@@ -120,9 +120,9 @@ int read_meta(Mech* m, const std::string& path)
   return parse_meta(m, f, path);
 }
 
-// Launch helper: packs the kernel arguments; 1-D grid, one state per thread, in workgroups
-// of the kernel's own launch bound (the generator picks it per kernel).
-// One lane per element of n_lanes (a state, or a queue slot times the lanes per state).
+// Launch helper: packs the kernel arguments; 1-D grid in workgroups of the kernel's own
+// launch bound (the generator picks it per kernel). n_lanes counts states (or queue slots),
+// one lane each; a kernel that exports its lanes per state gets that many per state.
 int launch(Mech* m, const char* name, long long n_lanes, void* stream, void** args)
 {
   auto it = m->fn.find(name);
