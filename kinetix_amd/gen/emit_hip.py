@@ -1000,8 +1000,21 @@ def _emit_bk2_fp64(t, info, pstate=False, block=BLOCK, queue=False):
 # --------------------------------------------------------------------------------------
 
 def _emit_thermo_fp64(t, info):
+    """thermoCoeffs (thermoCoeffs.okl:1-41, thermodynamics.py:41-62): rho, cp_k and rho*cp.
+    The NASA-7 c_p coefficients of both pieces sit in an LDS table read at a per-lane offset
+    (T > T_mid picks the piece without a branch), as in the production-rate kernels: with the
+    coefficients as literals behind a per-species select, the compiler materialised the selects
+    early -- 260 VGPRs at 53 species, 506 spilled at 160. One pass over the species: Mbar
+    factors out of the mean, sum_k c_k y_k Mbar = Mbar sum_k c_k y_k, so no per-species value
+    stays live past its species; a scheduling barrier every 8 species keeps the compiler from
+    hoisting all N loads of Y to the top."""
     N = int(t['n_species'])
     S = _Src()
+    CP = np.zeros((2, N, 8))
+    for k in range(N):
+        CP[0, k, :5], CP[1, k, :5] = t['cp_hi'][k], t['cp_lo'][k]
+    S(f'__device__ const double kx_thermo_fp64_cp[{CP.size}] = {{' +
+      ', '.join(_literal(v) for v in CP.reshape(-1)) + '};', 0)
     S(f'extern "C" __global__ void __launch_bounds__({BLOCK})', 0)
     S('kx_thermo_fp64(const int n_states, const long long offsetT, const long long offset,', 0)
     S('               const double pressure_R, const double* __restrict__ state,', 0)
@@ -1009,6 +1022,10 @@ def _emit_thermo_fp64(t, info):
     S('               double* __restrict__ rhoCp, const double Tref)', 0)
     S('{', 0)
     S(POOL_MARK)
+    S(f'__shared__ double cp_s[{CP.size}];')
+    for line in lds_stage('cp_s', 'kx_thermo_fp64_cp', CP.size, BLOCK):
+        S(line)
+    S('__syncthreads();')
     S(f'const long long id = (long long)blockIdx.x * {BLOCK} + threadIdx.x;')
     S('if (id >= n_states) return;')
     S('const double T = Tref * state[id];')
@@ -1018,23 +1035,23 @@ def _emit_thermo_fp64(t, info):
     S('const double T4 = T * T * T * T;')
     rcpW = t['rcpW']
     R = float(t['R'])
+    tmid = t['T_mid']
+    for tm in sorted({float(x) for x in tmid}):
+        S(f'const double* C{_tag(tm)} = cp_s + (T > {lit(tm)} ? 0 : {N * 8});')
+    S('double rcpMbar = 0.0, cy = 0.0;')
     for k in range(N):
-        S(f'const double y{k} = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
-    S('const double rcpMbar = ' + ' + '.join(f'y{k}' for k in range(N)) + ';')
+        if k % 8 == 0 and k:
+            S('__builtin_amdgcn_sched_barrier(0);')
+        S(f'{{ const double y = fmax(0.0, state[id + offsetT + {k} * offset]) * {lit(rcpW[k])};')
+        S('  rcpMbar += y;')
+        S(f'  const double* a = C{_tag(tmid[k])} + {8 * k};')
+        S('  const double c = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;')
+        S(f'  cp[{k} * offset + id] = c * {lit(R * rcpW[k])};')
+        S('  cy += c * y; }')
     S('const double Mbar = 1.0 / rcpMbar;')
     S('const double rho = pressure_R * rcpT * Mbar;')
     S('rho_out[id] = rho;')
-    S('double mean = 0.0;')
-    tmid = t['T_mid']
-    for tm in sorted({float(x) for x in tmid}):
-        S(f'const bool hi{_tag(tm)} = T > {lit(tm)};')
-    for k in range(N):
-        lo, hi = t['cp_lo'][k], t['cp_hi'][k]
-        def cpoly(a):
-            return f'{lit(a[0])} + {lit(a[1])} * T + {lit(a[2])} * T2 + {lit(a[3])} * T3 + {lit(a[4])} * T4'
-        S(f'{{ const double c = hi{_tag(tmid[k])} ? ({cpoly(hi)}) : ({cpoly(lo)});')
-        S(f'  cp[{k} * offset + id] = c * {lit(R * rcpW[k])};')
-        S(f'  mean += c * y{k} * Mbar; }}')
+    S('const double mean = cy * Mbar;')
     S(f'rhoCp[id] = rho * (mean * {lit(R)} * rcpMbar);')
     S('}', 0)
     return S.text()
