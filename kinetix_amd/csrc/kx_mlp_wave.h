@@ -56,6 +56,12 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// Weights and biases are read through global-address-space pointers (global_load, counted by
+// vmcnt only); through generic pointers they become flat loads, which also count against
+// lgkmcnt, so every wait for an LDS read would wait for the weight prefetches too.
+typedef const u32x4 __attribute__((address_space(1))) gu32x4;
+typedef const float __attribute__((address_space(1))) gfloat;
+typedef const f32x4 __attribute__((address_space(1))) gf32x4;
 
 __device__ __forceinline__ unsigned pack2(float a, float b)
 {
@@ -131,16 +137,16 @@ __device__ __forceinline__ void gelu_block(const f32x4 (&acc)[SB], Act<kH / 32, 
 }
 
 template <int KB>
-__device__ __forceinline__ void load_block(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+__device__ __forceinline__ void load_block(gu32x4* __restrict__ wf, gfloat* __restrict__ bias,
                                            int ob, int lane, u32x4 (&a)[KB], f32x4& b)
 {
-  b = *(const f32x4*)(bias + 16 * ob + 4 * (lane >> 4));
+  b = *(gf32x4*)(bias + 16 * ob + 4 * (lane >> 4));
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) a[kb] = wf[(ob * KB + kb) * 64 + lane];
 }
 
 template <int KB, int SB>
-__device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+__device__ __forceinline__ void hidden_layer(gu32x4* __restrict__ wf, gfloat* __restrict__ bias,
                                              const Act<KB, SB>& in, Act<kH / 32, SB>& out, int lane)
 {
   constexpr int OB = kH / 16;
@@ -165,18 +171,18 @@ __device__ __forceinline__ void hidden_layer(const u32x4* __restrict__ wf, const
 
 // y[ob][sb][e]: output feature 16 ob + 4 (lane / 16) + e of state 16 sb + lane % 16
 template <int SB>
-__device__ __forceinline__ void output_layer(const u32x4* __restrict__ wf, const float* __restrict__ bias,
+__device__ __forceinline__ void output_layer(gu32x4* __restrict__ wf, gfloat* __restrict__ bias,
                                              const Act<kH / 32, SB>& in, f32x4 (&y)[kOut / 16][SB], int lane)
 {
   const int g = lane >> 4;
   u32x4 a_cur[kH / 32], a_nxt[kH / 32];
-  f32x4 b_cur = *(const f32x4*)(bias + 4 * g), b_nxt;
+  f32x4 b_cur = *(gf32x4*)(bias + 4 * g), b_nxt;
 #pragma unroll
   for (int kb = 0; kb < kH / 32; ++kb) a_cur[kb] = wf[kb * 64 + lane];
 #pragma unroll
   for (int ob = 0; ob < kOut / 16; ++ob) {
     if (ob + 1 < kOut / 16) {
-      b_nxt = *(const f32x4*)(bias + 16 * (ob + 1) + 4 * g);
+      b_nxt = *(gf32x4*)(bias + 16 * (ob + 1) + 4 * g);
 #pragma unroll
       for (int kb = 0; kb < kH / 32; ++kb) a_nxt[kb] = wf[((ob + 1) * (kH / 32) + kb) * 64 + lane];
     }
@@ -204,14 +210,15 @@ __device__ __forceinline__ void forward(const uint16_t* __restrict__ wfrag, cons
   // wave (passes over 32 states), the weight loads are otherwise loop-invariant and the
   // compiler hoists or merges all 320 fragments (1,280 VGPRs) across the passes.
   asm volatile("" : "+s"(wfrag), "+s"(p));
-  const u32x4* wf = (const u32x4*)wfrag;
+  gu32x4* wf = (gu32x4*)wfrag;           // back to global: the asm copies are generic pointers
+  gfloat* pg = (gfloat*)p;
   Act<kIn / 32, SB> x0;
   load_input<XS, SB>(xs, x0, lane);
   Act<kH / 32, SB> h1, h2;
-  hidden_layer<kIn / 32>(wf + kF1 / 8, p + kB1, x0, h1, lane);
-  hidden_layer<kH / 32>(wf + kF2 / 8, p + kB2, h1, h2, lane);
-  hidden_layer<kH / 32>(wf + kF3 / 8, p + kB3, h2, h1, lane);
-  output_layer(wf + kF4 / 8, p + kB4, h1, y, lane);
+  hidden_layer<kIn / 32>(wf + kF1 / 8, pg + kB1, x0, h1, lane);
+  hidden_layer<kH / 32>(wf + kF2 / 8, pg + kB2, h1, h2, lane);
+  hidden_layer<kH / 32>(wf + kF3 / 8, pg + kB3, h2, h1, lane);
+  output_layer(wf + kF4 / 8, pg + kB4, h1, y, lane);
 }
 
 }  // namespace kxmlp

@@ -154,16 +154,23 @@ def emit(t, info, passes=2):
                 S(f'bl[{k + 2}] = lT15p + {expr};')
 
     def hull_and_inputs(store):
-        """out = 1 if any raw input lies outside the hull box (or is NaN); store(i, expr)
-        receives the standardised input i: (raw - mean) times the hardware reciprocal of the
-        standard deviation (~1 ulp FP32, far below the BF16 rounding the MLP input takes;
-        a correctly rounded divide costs ~10 instructions per input)."""
-        S('int out = 0;')
+        """out = 1 if any raw input lies outside the hull box (or is NaN or infinite); store(i,
+        expr) receives the standardised input i: (raw - mean) times the hardware reciprocal of
+        the standard deviation (~1 ulp FP32, far below the BF16 rounding the MLP input takes;
+        a correctly rounded divide costs ~10 instructions per input).
+
+        The box test is min_i min(raw_i - lo_i, hi_i - raw_i) >= 0 in floating point (with
+        FP32 denormals kept, x - y >= 0 exactly when x >= y), and NaN/infinity show up in
+        sum_i raw_i * 0: the same flag as per-input compares, without the 3 (N + 2) 64-bit
+        compare masks the compiler otherwise hoists and spills to VGPR lanes."""
+        S('float hmin = 1.0f, hnan = 0.0f;')
         S('#pragma unroll')
         S(f'for (int i = 0; i < {N + 2}; ++i) {{')
-        S(f'  out |= (raw[i] < norm[{2 * IN} + i]) | (raw[i] > norm[{3 * IN} + i]) | (raw[i] != raw[i]);')
+        S(f'  hmin = fminf(hmin, fminf(raw[i] - norm[{2 * IN} + i], norm[{3 * IN} + i] - raw[i]));')
+        S('  hnan += raw[i] * 0.0f;')
         S('  ' + store('i', f'(raw[i] - norm[i]) * __builtin_amdgcn_rcpf(norm[{IN} + i])'))
         S('}')
+        S('const int out = !(hmin >= 0.0f) | (hnan != hnan);')
 
     def rcp_mbar():
         S('double rcpMbar = 0.0;')

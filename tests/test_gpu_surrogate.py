@@ -248,6 +248,47 @@ def test_fused_kernel_matches_composed_path(trained, gri, n):
         assert float(((x_ - y_).abs() / y_.abs()).max()) <= 1e-5
 
 
+def test_hull_flags_at_the_box_edges(trained, gri):
+    """The out-of-hull flag against the training box evaluated on the host, on mass-fraction
+    inputs (input k + 2 is Y_k rounded to FP32): Y_k exactly on either bound is inside, one
+    FP32 ulp beyond either bound is outside, and NaN or +-inf flags the state. The feature kernel is checked state by state, the fused
+    kernel by its out-of-hull count."""
+    info, t = gri
+    kx, sur, _ = trained
+    n = 64
+    s, p = states.sample_transport_set(n, info['species'], t['W'], seed=31)
+    norm = sur.norm.cpu().numpy()
+    lo, hi = norm[2 * IN:3 * IN], norm[3 * IN:4 * IN]
+
+    def flags_of(st):
+        ds = torch.from_numpy(np.ascontiguousarray(st)).to(dev).reshape(-1)
+        dp = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+        _, _, fl = sur.features(n, n, n, dp, ds)
+        return fl.cpu().numpy(), ds, dp
+
+    f0, _, _ = flags_of(s)
+    inside = np.flatnonzero(f0 == 0)
+    assert inside.size >= 8
+    k = int(np.argmax(np.where(hi[2:2 + kx.nSpecies()] < 0.5, hi[2:2 + kx.nSpecies()], -1.0)))
+    assert 0.0 < hi[k + 2] < 0.5 and lo[k + 2] < hi[k + 2]
+    up = float(np.nextafter(np.float32(hi[k + 2]), np.float32(np.inf)))
+    down = float(np.nextafter(np.float32(lo[k + 2]), np.float32(-np.inf)))
+    edits = [(float(hi[k + 2]), 0), (up, 1), (float(lo[k + 2]), 0), (down, 1),
+             (np.nan, 1), (np.inf, 1), (-np.inf, 1)]
+    s2 = s.copy()
+    want = f0.copy()
+    for (v, w), j in zip(edits, inside):
+        s2[1 + k, j] = v
+        want[j] = w
+    f2, ds, dp = flags_of(s2)
+    assert np.array_equal(f2, want)
+    z = lambda m: torch.full((m,), np.nan, dtype=torch.float64, device=dev)
+    N = kx.nSpecies()
+    res = sur.infer(n, n, n, dp, ds, z(n), z(n), z(N * n), z(N * n), audit_stride=0)
+    torch.cuda.synchronize()
+    assert res['out_of_hull'] == int(want.sum())
+
+
 def test_out_of_hull_beyond_one_compact_pass(trained, gri):
     """Most states out of hull (4700 of 5000; the queue then holds far more than the audit):
     every one must equal the plain kernel's result."""
