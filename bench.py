@@ -223,6 +223,9 @@ def main():
             pass
         if sur is not None:
             bk2(); torch.cuda.synchronize(); launched.append('the surrogate')
+        d_cp = torch.empty(N * n, dtype=torch.float64, device=dev)
+        kx.thermodynamicProps(n, n, n, 1.0, d_state, d_visc, d_cp, d_cond)
+        torch.cuda.synchronize(); launched.append('kx_thermo_fp64')
         print(f'profile-only: copy {src.numel() * 8} B, {", ".join(launched)} over {n} states')
         return 0
 
@@ -388,6 +391,16 @@ def main():
                     'flops_per_state': mlp_flops(H), 'bytes_per_state': SUR_BYTES(N),
                     'hbm_GBps': round(gbs, 1), 'hbm_frac': round(gbs / PEAK_HBM, 4),
                     'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
+        if name == 'kx_thermo_fp64':
+            c = counts[name]
+            gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
+            r = {'bound': 'hbm', 'pipe': 'HBM3E (8 TB/s)', 'achieved': round(gbs, 1),
+                 'peak': PEAK_HBM, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM, 4), 'traffic': tr,
+                 'flops_per_state': c['flops'], 'bytes_per_state': c['bytes'],
+                 'avg_launch_ms': round(ms, 4), 'states_per_launch': n}
+            if tr:
+                r['traffic_over_algorithmic'] = round(tr / (c['bytes'] * n), 3)
+            return r
         c = counts[name if name in counts else ('kx_bk2_fp64' if 'bk2' in name else 'kx_bk1_fp64')]
         fl = c['flops'] * n / (ms * 1e-3) / 1e12
         gbs = c['bytes'] * n / (ms * 1e-3) / 1e9
@@ -433,6 +446,15 @@ def main():
     else:
         kern[bk2_name] = roof(bk2_name, m2)
         bk2_kernel_ms = m2
+    # the thermodynamic-properties kernel (thermoCoeffs.okl): rho, cp_k, rho*cp, HBM-bound
+    d_rho = torch.empty(n, dtype=torch.float64, device=dev)
+    d_rhocp = torch.empty_like(d_rho)
+    d_cp = torch.empty(N * n, dtype=torch.float64, device=dev)
+
+    def thermo():
+        kx.thermodynamicProps(n, n, n, 1.0, d_state, d_rho, d_cp, d_rhocp)
+    kern['kx_thermo_fp64'] = roof('kx_thermo_fp64', timed(thermo))
+    del d_rho, d_rhocp, d_cp
     bk2_kernel = 'kx_sur_infer' if sur is not None else bk2_name
     dom = bk1_name if m1 >= bk2_kernel_ms else bk2_kernel
     roofline = dict(kern[dom]); roofline['kernel'] = dom
