@@ -188,3 +188,15 @@ def test_tails_and_strided_views(setup, n):
     assert cond_err(R[:, :n], r0) < 1e-12 and cond_err(D[:, :n], d0) < 1e-12
     assert np.all(np.isnan(R[:, n:])) and np.all(np.isnan(D[:, n:]))
     assert np.all(np.isnan(v.cpu().numpy()[n:])) and np.all(np.isnan(c.cpu().numpy()[n:]))
+    # thermodynamic properties on the same strided states (coefficient table staged in LDS
+    # before the tail check)
+    rho, rhocp, cp = z(st), z(st), z(N * st)
+    kx.thermodynamicProps(n, st, st, 1.0, ds, rho, cp, rhocp)
+    torch.cuda.synchronize()
+    rho0, cp0, rhocp0 = orc.thermo(s, kx.refPressure())
+    CP = cp.cpu().numpy().reshape(N, st)
+    assert cond_err(rho.cpu().numpy()[:n], rho0) < 1e-12
+    assert cond_err(CP[:, :n], cp0) < 1e-12
+    assert cond_err(rhocp.cpu().numpy()[:n], rhocp0) < 1e-12
+    assert np.all(np.isnan(CP[:, n:]))
+    assert np.all(np.isnan(rho.cpu().numpy()[n:])) and np.all(np.isnan(rhocp.cpu().numpy()[n:]))
