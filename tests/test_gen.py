@@ -430,3 +430,44 @@ def test_tile_transport_tables_vs_oracle(gri, oracle):
         seen = [k * (N - 1) - k * (k - 1) // 2 - k - 1 + j for k in range(N) for j in range(k + 1, N)]
         assert seen == list(range(N * (N - 1) // 2))
     assert BT.lds_bytes({'n_species': 160}) == 160 * 1024
+
+
+def test_thermo_table_vs_oracle(gri, oracle):
+    """The thermodynamic kernel's coefficient table (both NASA-7 c_p pieces of every species in
+    rows of 8 doubles, high piece first) read back from the generated source and evaluated with
+    the kernel's index arithmetic -- base offset 0 when T > T_mid, else 8 N; species k at
+    8 k -- against the oracle's c_p, on both sides of and exactly at every T_mid (the low piece
+    at equality, as in thermoCoeffs). The same table with its pieces swapped fails."""
+    import re
+    from kinetix_amd import states
+    from kinetix_amd.gen import emit_hip as E
+    info, t = gri
+    N = int(t['n_species'])
+    src = E._emit_thermo_fp64(t, info)
+    m = re.search(r'kx_thermo_fp64_cp\[(\d+)\] = \{([^}]*)\}', src)
+    tab = np.array([float(v) for v in m.group(2).split(',')])
+    assert tab.size == int(m.group(1)) == 2 * N * 8
+    assert f'? 0 : {N * 8})' in src
+    tmid = np.asarray(t['T_mid'], dtype=np.float64)
+    Ts = [300.0, 750.0, 2500.0, 3500.0]
+    for tm in sorted(set(tmid.tolist())):
+        Ts += [np.nextafter(tm, 0.0), tm, np.nextafter(tm, np.inf), tm - 50.0, tm + 50.0]
+    T = np.array(Ts)
+    s, _ = states.sample_any(T.size, info['species'], t['W'], seed=23)
+    s[0] = T
+    _, cp0, _ = oracle.thermo(s, 101325.0)
+    R, rcpW = float(t['R']), np.asarray(t['rcpW'])
+
+    def kernel_cp(table):
+        out = np.empty((N, T.size))
+        for i, Ti in enumerate(T):
+            for k in range(N):
+                base = 0 if Ti > tmid[k] else 8 * N
+                a = table[base + 8 * k: base + 8 * k + 5]
+                c = a[0] + a[1] * Ti + a[2] * (Ti * Ti) + a[3] * (Ti * Ti * Ti) + a[4] * (Ti * Ti * Ti * Ti)
+                out[k, i] = c * (R * rcpW[k])
+        return out
+
+    assert np.max(np.abs(kernel_cp(tab) / cp0 - 1)) < 1e-12
+    swapped = np.concatenate([tab[8 * N:], tab[:8 * N]])
+    assert np.max(np.abs(kernel_cp(swapped) / cp0 - 1)) > 1e-4
