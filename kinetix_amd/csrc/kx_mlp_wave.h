@@ -200,8 +200,9 @@ __device__ __forceinline__ void output_layer(gu32x4* __restrict__ wf, gfloat* __
 }
 
 // Forward pass for 16 SB states (a wave's 64 at SB = 4). wf: fragment block (kFragTotal
-// bf16, 16-B aligned); p: FP32 parameter block (biases); xs: LDS rows of the standardised
-// inputs of the pass's first state onwards.
+// bf16, 16-B aligned); p: FP32 parameter block (biases); both in global memory (device
+// allocations: they are read through global-address-space pointers); xs: LDS rows of the
+// standardised inputs of the pass's first state onwards.
 template <int XS, int SB = 4>
 __device__ __forceinline__ void forward(const uint16_t* __restrict__ wfrag, const float* __restrict__ p,
                                         const uint16_t* xs, f32x4 (&y)[kOut / 16][SB], int lane)

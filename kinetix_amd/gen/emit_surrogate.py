@@ -161,8 +161,10 @@ def emit(t, info, passes=2):
 
         The box test is min_i min(raw_i - lo_i, hi_i - raw_i) >= 0 in floating point (with
         FP32 denormals kept, x - y >= 0 exactly when x >= y), and NaN/infinity show up in
-        sum_i raw_i * 0: the same flag as per-input compares, without the 3 (N + 2) 64-bit
-        compare masks the compiler otherwise hoists and spills to VGPR lanes."""
+        sum_i raw_i * 0. With finite bounds (every trained box) this is the same flag as
+        per-input compares, without the 3 (N + 2) 64-bit compare masks the compiler otherwise
+        hoists and spills to VGPR lanes; with the infinite bounds of the training pass's
+        identity normalisation it also flags infinite inputs, a flag that pass ignores."""
         S('float hmin = 1.0f, hnan = 0.0f;')
         S('#pragma unroll')
         S(f'for (int i = 0; i < {N + 2}; ++i) {{')
