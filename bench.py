@@ -303,6 +303,17 @@ def main():
             'rhoD_max_rel_vs_plain': float((d_rhoD / p_rhoD - 1).abs().nan_to_num(0.0).max())}
     except KinetixError as e:              # loop-form mechanisms have no MFMA form
         bk2_variants['fp64_mfma'] = {'kernel': 'kx_bk2_fp64_mfma', 'absent': str(e)}
+
+    def bk2_pstate():
+        kx.mixtureAvgTransportPropsPstate(n, n, n, d_p, d_state, d_visc, d_cond, d_rhoD)
+    bk2_pstate(); torch.cuda.synchronize()
+    ms_ps = timed(bk2_pstate)
+    bk2_variants['fp64_pstate'] = {
+        'kernel': 'kx_bk2_fp64_pstate', 'ms': round(ms_ps, 4),
+        'note': 'per-state pressure (the transport workload form), here every state at 1 atm',
+        'viscosity_max_rel_vs_plain': float((d_visc / p_visc - 1).abs().max()),
+        'conductivity_max_rel_vs_plain': float((d_cond / p_cond - 1).abs().max()),
+        'rhoD_max_rel_vs_plain': float((d_rhoD / p_rhoD - 1).abs().nan_to_num(0.0).max())}
     bk2_plain(); torch.cuda.synchronize()
     sur_err = None
     if sur is not None:
